@@ -1,0 +1,247 @@
+"""Benchmark of the GSO hot path on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c3]
+
+A step is one GSO step (pose transform + DFIRE scoring + luciferin, then
+neighbour selection + movement) over every glowworm of every swarm of the
+workload; inputs and state are resident in HBM before timing starts.  The
+headline form is the batched FP32 guard-band form; the plain FP64 form is
+timed on the same workload for comparison.  Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from sparkledock_amd import sdgso, workloads  # noqa: E402
+
+METRIC = "glowworm-pose evals/sec on 1 MI355X, plain FP64 vs batched FP32-guarded"
+FP32_PEAK_TFLOPS = 157.3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def hip_sync():
+    import torch
+    torch.cuda.synchronize()
+
+
+def timed_steps(run, n, graph):
+    """Steps on torch's current stream, bracketed by device synchronisation."""
+    hip_sync()
+    t0 = time.perf_counter()
+    run.steps(n, graph=graph)
+    run.sync()
+    hip_sync()
+    return time.perf_counter() - t0
+
+
+def precision_check(wl, n_per_complex=64, seed=5):
+    """FP32-guard vs FP64 (the FP64 form is bit-exact to the CPU oracle in the
+    parity tests) on a sample of poses of every complex: pair counts (bins)
+    and energies relative to the sum of |T| over in-cutoff pairs."""
+    rng = np.random.default_rng(seed)
+    pos = wl["pos"].reshape(-1, wl["complexes"][0].P)
+    G = wl["G"]
+    start = 0
+    bins_equal = 0
+    n = 0
+    worst_rel = 0.0
+    worst_rel_score = 0.0
+    tab_abs = np.abs(sdgso.synth_table(workloads.TABLE_SEED))
+    for cx, S, inp in zip(wl["complexes"], wl["n_swarms"], wl["inputs"]):
+        idx = start + rng.choice(S * G, size=min(n_per_complex, S * G), replace=False)
+        poses = pos[idx]
+        s64, _, p64 = cx.score(poses, sdgso.FP64)
+        s32, _, p32 = cx.score(poses, sdgso.FP32_GUARD)
+        cabs = sdgso.Complex(inp["r_xyz"], inp["r_type"], inp["l_xyz"], inp["l_type"], tab_abs,
+                             inp["lig_ref"], inp["r_modes"], inp["l_modes"])
+        sabs, _, _ = cabs.score(poses, sdgso.FP64)
+        cabs.close()
+        denom = np.maximum((4.7 - sabs) / 0.0157, 1.0)
+        e64, e32 = (4.7 - s64) / 0.0157, (4.7 - s32) / 0.0157
+        worst_rel = max(worst_rel, float(np.max(np.abs(e32 - e64) / denom)))
+        worst_rel_score = max(worst_rel_score,
+                              float(np.max(np.abs(s32 - s64) / np.maximum(np.abs(s64), 1e-300))))
+        bins_equal += int(np.sum(p64 == p32))
+        n += len(idx)
+        start += S * G
+    return dict(poses=n, pair_counts_equal=bins_equal, energy_rel_err_sum_abs=worst_rel,
+                score_rel_err=worst_rel_score)
+
+
+def cpu_baseline(wl, threads, target_s=12.0):
+    """The CPU oracle (FP64 restatement of the reference CPU step) on a bounded
+    sample: one full step (score, select, move) over `threads` swarms of g
+    glowworms of every complex; per-evaluation cost extrapolates to the whole
+    workload."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    G = wl["G"]
+    P = wl["complexes"][0].P
+    pos = wl["pos"].reshape(-1, G, P)
+    # calibrate g so that the sample takes about target_s
+    swarm0 = 0
+    cost = 0.0
+    samples = []
+    for inp, S in zip(wl["inputs"], wl["n_swarms"]):
+        cx = O.Complex(inp["r_xyz"], inp["r_type"], inp["l_xyz"], inp["l_type"], inp["table"],
+                       inp["lig_ref"], inp["r_modes"], inp["l_modes"], inp["r_mask"], inp["l_mask"])
+        samples.append((cx, swarm0, S))
+        cost += S * len(inp["r_xyz"]) * len(inp["l_xyz"])
+        swarm0 += S
+    # pairs per second of this host on all threads, from a small probe
+    cx, s0, S = samples[0]
+    ns = min(threads, S)
+    st = O.State(ns, 2, P, O.default_params(), pos[s0:s0 + ns, :2].copy())
+    u = np.full((ns, 1, 2), 0.5)
+    t0 = time.perf_counter()
+    O.lib().or_gso_step(cx.c, O.default_params(), st.st, O._p(np.ascontiguousarray(u)), 1, 0, 1,
+                        threads)
+    rate = ns * 2 * cx.nr * cx.nl / max(time.perf_counter() - t0, 1e-6)
+    # glowworms per sampled swarm so that the sample takes about target_s
+    pairs_per_g = sum(min(threads, S2) * cx2.nr * cx2.nl for cx2, _, S2 in samples)
+    g = int(max(1, min(G, target_s * rate / pairs_per_g)))
+    t_total = 0.0
+    evals = 0
+    step_time = 0.0  # extrapolated time of one full workload step
+    for cx, s0, S in samples:
+        ns = min(threads, S)
+        st = O.State(ns, g, P, O.default_params(), pos[s0:s0 + ns, :g].copy())
+        u = np.full((ns, 1, g), 0.5)
+        t0 = time.perf_counter()
+        O.lib().or_gso_step(cx.c, O.default_params(), st.st, O._p(np.ascontiguousarray(u)), 1, 0,
+                            1, threads)
+        dt = time.perf_counter() - t0
+        t_total += dt
+        evals += ns * g
+        step_time += dt / (ns * g) * S * G
+    total_evals = sum(S for _, _, S in samples) * G
+    return dict(value=total_evals / step_time, unit="evals/s", cores=threads, kind="port",
+                sample="one GSO step over %d swarms x %d glowworms of each of %d complexes "
+                       "(%d evals, %.1f s); rate extrapolated by per-complex evaluation cost"
+                       % (min(threads, min(S for _, _, S in samples)), g, len(samples), evals,
+                          t_total))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="c3", choices=sorted(workloads.CONFIGS))
+    ap.add_argument("--swarm-scale", type=float, default=1.0)
+    ap.add_argument("--fp64-steps", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the GSO path has no multi-GPU form in this build "
+              "(swarms would shard across GPUs with no exchange; see the design notes). "
+              "Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    if not torch.cuda.is_available():
+        sys.exit("no GPU visible")
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+
+    t0 = time.perf_counter()
+    wl = workloads.build(args.workload, swarm_scale=args.swarm_scale,
+                         steps=args.warmup + args.steps + 1)
+    n_glow = sum(wl["n_swarms"]) * wl["G"]
+    log("workload %s: %d complexes, %d glowworms, built in %.1f s" %
+        (args.workload, len(wl["complexes"]), n_glow, time.perf_counter() - t0))
+
+    # ---- batched FP32 guard-band form (headline)
+    run = sdgso.Run(wl["complexes"], wl["n_swarms"], wl["G"], wl["pos"], wl["stream"], wl["steps"],
+                    form=sdgso.FP32_GUARD)
+    run.set_stream(stream.cuda_stream)
+    run.steps(args.warmup)
+    run.sync()
+    run.profile(True)
+    dt = timed_steps(run, args.steps, graph=False)
+    prof = run.profile_read()
+    guard = run.guard_stats()
+    run.profile(False)
+    st = run.read()
+    pairs_per_launch = float(st["pairs"].sum())
+    score_ms = prof["score_ms"] / max(1, prof["score_launches"])
+    log("fp32 guard: %.2f ms/step, score kernel %.2f ms, pairs/launch %.3e, guard recomputes %d"
+        % (1e3 * dt / args.steps, score_ms, pairs_per_launch, guard["recomputed"]))
+    # graph replay of the same step
+    dtg = timed_steps(run, 1, graph=True) if run.k < wl["steps"] else None
+    value = n_glow * args.steps / dt
+
+    # ---- plain FP64 per-complex form on the same workload
+    fp64 = None
+    if not args.no_fp64 and args.fp64_steps > 0:
+        t64 = 0.0
+        for cx, S, k in zip(wl["complexes"], wl["n_swarms"], range(len(wl["complexes"]))):
+            off = sum(wl["n_swarms"][:k]) * wl["G"] * cx.P
+            pos_k = wl["pos"][off:off + S * wl["G"] * cx.P]
+            st_k = sdgso.stream(0, S, args.fp64_steps, wl["G"])
+            r64 = sdgso.Run(cx, S, wl["G"], pos_k, st_k, args.fp64_steps, form=sdgso.FP64)
+            r64.set_stream(stream.cuda_stream)
+            t64 += timed_steps(r64, args.fp64_steps, graph=False)
+            r64.close()
+        fp64 = dict(value=n_glow * args.fp64_steps / t64,
+                    ms_per_step=1e3 * t64 / args.fp64_steps, steps=args.fp64_steps,
+                    form="one run per complex, brute force over all pairs in FP64")
+        log("fp64 plain: %.1f ms/step" % fp64["ms_per_step"])
+
+    prec = precision_check(wl)
+    log("precision:", prec)
+
+    cpu = None
+    if not args.no_cpu:
+        threads = len(os.sched_getaffinity(0))
+        threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
+        cpu = cpu_baseline(wl, threads)
+        log("cpu baseline:", cpu)
+
+    flops = 8.0 * pairs_per_launch
+    achieved = flops / (score_ms * 1e-3) / 1e12
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "evals/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32 (FP64 guard-band recompute, Kahan sums)",
+        "data": "synthetic, seeded (random-walk globules, N(0,1)*0.5 table)",
+        "config": {"workload": args.workload, "complexes": len(wl["complexes"]),
+                   "swarms": int(sum(wl["n_swarms"])), "glowworms_per_swarm": wl["G"],
+                   "modes": list(wl["spec"]["modes"]), "guard_band_A2": 1e-4,
+                   "parallelism": "1 GPU, all swarms of all complexes in one run"},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "score32_kernel", "launch_ms": score_ms,
+                     "algorithmic": "8 FP32 flop per in-cutoff pair (0 < d2 <= 225), "
+                                    "%.4e pairs per launch" % pairs_per_launch},
+        "cpu_baseline": cpu,
+        "plain_fp64": fp64,
+        "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
+        "graph_ms_per_step": (1e3 * dtg) if dtg else None,
+        "precision": prec,
+        "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
+        "pairs_per_eval": pairs_per_launch / n_glow,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

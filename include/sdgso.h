@@ -1,0 +1,163 @@
+/*
+ * sdgso.h: C-ABI of the MI355X (gfx950) GSO docking hot path.
+ *
+ * One library, libsdgso.so, owns all device memory behind opaque handles and
+ * takes plain host pointers and sizes.  Every call returns an int status
+ * (SDGSO_OK = 0); sdgso_last_error() returns the message of the calling
+ * thread's last failure.  Handles are not thread-safe: use one per thread.
+ *
+ * Reference interfaces replaced (paths relative to the reference root):
+ *   sdgso_complex_create  <- the per-complex inputs prepare_gso_tasks() hands to
+ *                            the GSO engine: Complex coordinates/modes/mask,
+ *                            FastDifire table, dfire_objects_* types and the
+ *                            ligand reference point
+ *                            (source/dock/src/prepare_gso_tasks.cc:1151-1256;
+ *                             source/dock/include/prepare_gso_tasks.h:8-25)
+ *   sdgso_score_poses     <- cdfire_calculate_dfire(), the scoring plugin, with
+ *                            the pose transform in front of it
+ *                            (prepare_gso_tasks.cc:457-545, 1522-1615;
+ *                             prepare_gso_tasks.h:36-40)
+ *   sdgso_run_*           <- cal_gso_tasks_gpu()/cal_gso_tasks_cpu(): the GSO
+ *                            step loop over every glowworm of every swarm
+ *                            (source/dock/src/cuda/prepare_gso_task.cu:2479-2972;
+ *                             prepare_gso_tasks.cc:359-822; prepare_gso_tasks.h:18-25)
+ */
+#ifndef SDGSO_H
+#define SDGSO_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SDGSO_NTYPES 168
+#define SDGSO_NBINS 20
+#define SDGSO_TABLE_LEN (SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS)
+#define SDGSO_MAX_MODES 32
+
+enum {
+    SDGSO_OK = 0,
+    SDGSO_EINVAL = 1,  /* bad argument (size, range, null pointer) */
+    SDGSO_EHIP = 2,    /* HIP runtime failure or no usable gfx950 device */
+    SDGSO_ENOMEM = 3,  /* device or host allocation failed */
+    SDGSO_ESTATE = 4   /* call out of order (e.g. stepping past the stream) */
+};
+
+/* Scoring forms. */
+enum {
+    SDGSO_FP64 = 0,       /* plain FP64 brute force over all Nr*Nl pairs */
+    SDGSO_FP32_GUARD = 1  /* FP32 distances on receptor tiles, FP64 recompute of
+                             pairs within the guard band of a bin boundary,
+                             compensated FP32 energy sums */
+};
+
+/* GSO constants (prepare_gso_tasks.cc:386-400; steps :783, 791-794). */
+typedef struct {
+    double rho, gamma, beta;
+    double vision0, max_vision, luciferin0;
+    int max_neighbors;
+    double step_translation, step_rotation, step_nmodes;
+} sdgso_params;
+
+typedef struct sdgso_complex_s *sdgso_complex;
+typedef struct sdgso_run_s *sdgso_run;
+
+const char *sdgso_last_error(void);
+int sdgso_version(void);
+void sdgso_default_params(sdgso_params *p);
+
+/* Receptor/ligand of one docking problem.
+ *   rec_xyz [n_rec][3] f64, rec_type [n_rec] in [0,168), rec_mask [n_rec] (1 =
+ *   moved by the normal modes, as complex.cc:149-209 sets it),
+ *   rec_modes [n_rec_modes][n_rec][3] f64 (NULL when n_rec_modes == 0); the
+ *   same for the ligand; lig_ref [3] the ligand reference point;
+ *   table [168*168*20] f32, index a*3360 + b*20 + bin. */
+int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type,
+                         const int *rec_mask, int n_rec_modes, const double *rec_modes,
+                         int n_lig, const double *lig_xyz, const int *lig_type,
+                         const int *lig_mask, int n_lig_modes, const double *lig_modes,
+                         const double *lig_ref, const float *table, sdgso_complex *out);
+int sdgso_complex_destroy(sdgso_complex c);
+
+/* Score n_poses poses [n][7+nr+nl] = (tx,ty,tz, qw,qx,qy,qz, rec extents,
+ * lig extents); out_score [n]; optional out_ref [n][3] ligand reference points,
+ * optional out_pairs [n] in-cutoff pair counts (NULL to skip).
+ * guard_band in A^2 (SDGSO_FP32_GUARD only; <= 0 selects 1e-4). */
+int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int form,
+                      double guard_band, double *out_score, double *out_ref,
+                      int64_t *out_pairs);
+
+/* A GSO run over one or more complexes sharing the glowworm count and pose
+ * length.  Complex k owns swarms [sum_{j<k} n_swarms[j], ...).
+ *   pos    [sum S][G][P] initial poses
+ *   stream [sum S][n_steps][G] the uniform draws of each swarm and step
+ *          (each complex's segment is its own mt19937 stream, swarm-major)
+ * The run keeps all state resident on the device. */
+int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n_swarms,
+                     int n_glowworms, const double *pos, const double *stream, int n_steps,
+                     const sdgso_params *params, int form, double guard_band, sdgso_run *out);
+int sdgso_run_destroy(sdgso_run r);
+
+/* Attach a HIP stream (hipStream_t) for all later launches of this run; NULL =
+ * the library's own stream. */
+int sdgso_run_set_stream(sdgso_run r, void *hip_stream);
+
+/* Advance n steps (asynchronous on the run's stream).  use_graph != 0 replays
+ * the step as a captured hipGraph. */
+int sdgso_run_steps(sdgso_run r, int n, int use_graph);
+
+/* Phases of one step, for single-phase tests: bit 0 score + luciferin,
+ * bit 1 neighbour selection, bit 2 move + vision; the step counter advances
+ * when bit 2 runs. */
+int sdgso_run_phases(sdgso_run r, int mask);
+
+int sdgso_run_sync(sdgso_run r);
+
+/* Number of steps taken so far. */
+int sdgso_run_step_index(sdgso_run r);
+
+/* Copy state back (synchronises).  Any pointer may be NULL.
+ *   pos [N][P], luciferin/score/vision [N], nnei/moved/select [N],
+ *   ref [N][3], pairs [N] (in-cutoff pairs of the last scoring), N = sum S * G. */
+int sdgso_run_read(sdgso_run r, double *pos, double *luciferin, double *score, double *vision,
+                   int *nnei, int *moved, int *select, double *ref, int64_t *pairs);
+
+/* Overwrite state (synchronises).  NULL pointers leave that field unchanged. */
+int sdgso_run_write(sdgso_run r, const double *pos, const double *luciferin, const double *score,
+                    const double *vision, const int *moved, const double *ref);
+
+/* Per-phase HIP-event timing on the run's stream.  enable != 0 starts
+ * recording; read returns totals in milliseconds and launch counts since the
+ * last enable: times[0] scoring, times[1] select + move. */
+int sdgso_run_profile(sdgso_run r, int enable);
+int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches);
+
+/* Guard-band statistics of the FP32 form since the last enable of profiling:
+ * stats[0] pairs recomputed in FP64. */
+int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
+
+/* ---- host utilities (no device use) ---- */
+
+/* The reference's random stream: u[s][k][g] = successive
+ * uniform_real_distribution<double>(0,1) draws from mt19937(seed)
+ * (prepare_gso_tasks.cc:411-427). */
+int sdgso_stream(uint32_t seed, int n_swarms, int n_steps, int n_glowworms, double *u);
+
+/* Seeded synthetic complex: random-walk chains (3.8 A steps) in globules of
+ * radius (N*11.4)^(1/3), uniform types over 168, orthonormal random modes.
+ * Any output pointer may be NULL. */
+int sdgso_synth_molecule(uint64_t seed, int n_atoms, int n_modes, double *xyz, int *type,
+                         double *modes);
+/* 168*168*20 f32 table drawn N(0,1)*0.5. */
+int sdgso_synth_table(uint64_t seed, float *table);
+/* Initial poses [S][G][7+nr+nl] around swarm centres on a sphere of radius
+ * centre_radius (translations in a swarm_radius ball, Shoemake quaternions,
+ * extents U[-1,1]; cal_anm.cc:845-892). */
+int sdgso_synth_poses(uint64_t seed, int n_swarms, int n_glowworms, int n_rec_modes,
+                      int n_lig_modes, double centre_radius, double swarm_radius, double *pos);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

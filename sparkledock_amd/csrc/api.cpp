@@ -1,0 +1,853 @@
+// C-ABI entry points of libsdgso.so (declared in include/sdgso.h).
+//
+// Host responsibilities: argument validation, the per-complex preprocessing
+// (SoA layouts, mask folded into the modes, Morton-ordered receptor tiles and
+// ligand blocks for the FP32 guard form), device residency of every array, the
+// step sequence on one HIP stream, optional hipGraph replay of a step, and
+// HIP-event timing of the phases.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "sd_internal.h"
+
+namespace sd {
+
+static thread_local std::string g_error;
+
+void set_error(const std::string &msg) { g_error = msg; }
+
+struct Failure {
+    int code;
+};
+
+[[noreturn]] static void fail(int code, const std::string &msg) {
+    set_error(msg);
+    throw Failure{code};
+}
+
+#define HIPCHK(expr)                                                                     \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            fail(SDGSO_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_));       \
+    } while (0)
+
+template <class F>
+static int guarded(F &&f) {
+    try {
+        f();
+        return SDGSO_OK;
+    } catch (const Failure &e) {
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host allocation failed");
+        return SDGSO_ENOMEM;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return SDGSO_EINVAL;
+    }
+}
+
+// ---------------------------------------------------------------- device setup
+static int g_device = -1;
+
+static void ensure_device() {
+    if (g_device >= 0) {
+        HIPCHK(hipSetDevice(g_device));
+        return;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0)
+        fail(SDGSO_EHIP, "no HIP device visible (libsdgso needs an MI355X / gfx950)");
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    hipDeviceProp_t prop;
+    HIPCHK(hipGetDeviceProperties(&prop, dev));
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        fail(SDGSO_EHIP, std::string("device is ") + prop.gcnArchName +
+                             "; libsdgso is built for gfx950 only");
+    g_device = dev;
+}
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        reset();
+        if (count == 0) return;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            fail(SDGSO_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
+        }
+        n = count;
+    }
+    void upload(const T *src, size_t count) {
+        alloc(count);
+        if (count) HIPCHK(hipMemcpy(p, src, count * sizeof(T), hipMemcpyHostToDevice));
+    }
+    void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
+};
+
+// ---------------------------------------------------------------- Morton order
+static uint64_t spread3(uint64_t v) {
+    v &= 0x1fffff;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+}
+
+// Atoms ordered along a Morton curve over their bounding box.
+static std::vector<int> morton_order(const std::vector<double> &xyz, int n) {
+    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) {
+            lo[c] = std::min(lo[c], xyz[3 * i + c]);
+            hi[c] = std::max(hi[c], xyz[3 * i + c]);
+        }
+    double ext = 1e-9;
+    for (int c = 0; c < 3; c++) ext = std::max(ext, hi[c] - lo[c]);
+    std::vector<std::pair<uint64_t, int>> key(n);
+    for (int i = 0; i < n; i++) {
+        uint64_t code = 0;
+        for (int c = 0; c < 3; c++) {
+            const uint64_t q = (uint64_t)((xyz[3 * i + c] - lo[c]) / ext * 1048575.0);
+            code |= spread3(q) << c;
+        }
+        key[i] = {code, i};
+    }
+    std::sort(key.begin(), key.end());
+    std::vector<int> order(n);
+    for (int i = 0; i < n; i++) order[i] = key[i].second;
+    return order;
+}
+
+}  // namespace sd
+
+using namespace sd;
+
+// ---------------------------------------------------------------- complex
+struct sdgso_complex_s {
+    int nr = 0, nl = 0, nmr = 0, nml = 0;
+    // host copies kept for argument checks and statistics
+    std::vector<double> r_xyz, l_xyz;
+    DevBuf<double> r0, rmodes, l0, lmodes;
+    DevBuf<int> rtype, ltype;
+    DevBuf<float> table;
+    DevBuf<float4> tiles, lblocks;
+    DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
+    DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
+    DevBuf<int> rtype_s, rperm, lperm;
+    DevComplex dev{};          // descriptor with device pointers
+    double max_rec_extent = 0; // |xyz| maxima, for the guard-band bound
+};
+
+struct sdgso_run_s {
+    int n_complex = 0;
+    std::vector<sdgso_complex> cxs;
+    std::vector<int> n_swarms;
+    int S = 0, G = 0, P = 0, N = 0, nmr = 0, nml = 0;
+    int n_steps = 0, k = 0;
+    int form = SDGSO_FP64;
+    float band = 1e-4f;
+    sdgso_params params{};
+    int max_nl = 0, max_lblocks = 0;
+    DevBuf<DevComplex> dcx;
+    DevBuf<int> swarm_cx;
+    DevBuf<double> pos, luciferin, score, vision, ref, stream, partial;
+    DevBuf<int> nnei, moved, select, step, work;
+    DevBuf<int64_t> pairs;
+    DevBuf<long long> partial_pairs;
+    DevBuf<int> d_k;
+    DevBuf<unsigned long long> guard_count;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream_h = nullptr;
+    hipGraphExec_t graph = nullptr;
+    bool profiling = false;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_score, ev_move;
+    double t_score = 0, t_move = 0;
+    int64_t n_score = 0, n_move = 0;
+};
+
+static void check_table_and_types(const float *table, const int *rt, int nr, const int *lt, int nl) {
+    if (!table) fail(SDGSO_EINVAL, "table is null");
+    for (int i = 0; i < nr; i++)
+        if (rt[i] < 0 || rt[i] >= SDGSO_NTYPES)
+            fail(SDGSO_EINVAL, "receptor type out of [0,168) at atom " + std::to_string(i));
+    for (int i = 0; i < nl; i++)
+        if (lt[i] < 0 || lt[i] >= SDGSO_NTYPES)
+            fail(SDGSO_EINVAL, "ligand type out of [0,168) at atom " + std::to_string(i));
+}
+
+// SoA [3][n] coordinates, [nm][3][n] modes with the mask folded in (zero rows
+// for atoms the modes do not move: R += 0*e leaves R unchanged exactly).
+static void soa_inputs(int n, const double *xyz, const int *mask, int nm, const double *modes,
+                       std::vector<double> &soa, std::vector<double> &msoa) {
+    soa.assign(3 * (size_t)n, 0.0);
+    for (int i = 0; i < n; i++)
+        for (int c = 0; c < 3; c++) soa[(size_t)c * n + i] = xyz[3 * (size_t)i + c];
+    msoa.assign((size_t)nm * 3 * n, 0.0);
+    for (int m = 0; m < nm; m++)
+        for (int i = 0; i < n; i++) {
+            if (mask && mask[i] != 1) continue;
+            for (int c = 0; c < 3; c++)
+                msoa[((size_t)m * 3 + c) * n + i] = modes[((size_t)m * n + i) * 3 + c];
+        }
+}
+
+static void build_tiles(sdgso_complex c, const double *xyz, const int *type, const std::vector<double> &msoa) {
+    const int n = c->nr, nm = c->nmr;
+    std::vector<double> v(xyz, xyz + 3 * (size_t)n);
+    std::vector<int> order = morton_order(v, n);
+    const int ntiles = (n + kTile - 1) / kTile;
+    const size_t slots = (size_t)ntiles * kTile;
+    std::vector<float4> tiles(ntiles);
+    std::vector<double> rrel(3 * slots, 0.0);
+    std::vector<float> m32((size_t)nm * 3 * slots, 0.0f);
+    std::vector<int> ts(slots, 0), perm(slots, -1);
+    for (int t = 0; t < ntiles; t++) {
+        const int a0 = t * kTile, a1 = std::min(n, a0 + kTile);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int s = a0; s < a1; s++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], xyz[3 * (size_t)order[s] + k]);
+                hi[k] = std::max(hi[k], xyz[3 * (size_t)order[s] + k]);
+            }
+        // centre rounded to a float so that kernels and host agree exactly
+        float cf[3];
+        for (int k = 0; k < 3; k++) cf[k] = (float)(0.5 * (lo[k] + hi[k]));
+        double rad = 0;
+        for (int s = a0; s < a1; s++) {
+            double d2 = 0;
+            for (int k = 0; k < 3; k++) {
+                const double d = xyz[3 * (size_t)order[s] + k] - (double)cf[k];
+                d2 += d * d;
+            }
+            rad = std::max(rad, std::sqrt(d2));
+        }
+        tiles[t] = make_float4(cf[0], cf[1], cf[2], (float)(rad * (1 + 1e-6) + 1e-4));
+        for (int s = a0; s < t * kTile + kTile; s++) {
+            if (s < a1) {
+                const int i = order[s];
+                perm[s] = i;
+                ts[s] = type[i] * SDGSO_NBINS * SDGSO_NTYPES;
+                for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = xyz[3 * (size_t)i + k] - (double)cf[k];
+                for (int m = 0; m < nm; m++)
+                    for (int k = 0; k < 3; k++)
+                        m32[((size_t)m * 3 + k) * slots + s] = (float)msoa[((size_t)m * 3 + k) * n + i];
+            } else {
+                // padding slot: far away, type 0, zero modes
+                for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = 1.0e4;
+            }
+        }
+    }
+    for (int m = 0; m < nm; m++) {
+        double mx = 0;
+        for (int i = 0; i < n; i++) {
+            double d2 = 0;
+            for (int k = 0; k < 3; k++) {
+                const double x = msoa[((size_t)m * 3 + k) * n + i];
+                d2 += x * x;
+            }
+            mx = std::max(mx, std::sqrt(d2));
+        }
+        c->dev.rmode_max[m] = (float)(mx * (1 + 1e-5));
+    }
+    c->tiles.upload(tiles);
+    c->rrel.upload(rrel);
+    c->rmod32.upload(m32);
+    c->rtype_s.upload(ts);
+    c->rperm.upload(perm);
+    c->dev.ntiles = ntiles;
+}
+
+static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<double> &msoa) {
+    const int n = c->nl, nm = c->nml;
+    std::vector<double> v(xyz, xyz + 3 * (size_t)n);
+    std::vector<int> order = morton_order(v, n);
+    const int nb = (n + kBlock - 1) / kBlock;
+    std::vector<float4> blocks(nb);
+    std::vector<int> perm((size_t)nb * kBlock, -1);
+    for (int b = 0; b < nb; b++) {
+        const int a0 = b * kBlock, a1 = std::min(n, a0 + kBlock);
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int s = a0; s < a1; s++)
+            for (int k = 0; k < 3; k++) {
+                lo[k] = std::min(lo[k], xyz[3 * (size_t)order[s] + k]);
+                hi[k] = std::max(hi[k], xyz[3 * (size_t)order[s] + k]);
+            }
+        float cf[3];
+        for (int k = 0; k < 3; k++) cf[k] = (float)(0.5 * (lo[k] + hi[k]));
+        double rad = 0;
+        for (int s = a0; s < a1; s++) {
+            double d2 = 0;
+            for (int k = 0; k < 3; k++) {
+                const double d = xyz[3 * (size_t)order[s] + k] - (double)cf[k];
+                d2 += d * d;
+            }
+            rad = std::max(rad, std::sqrt(d2));
+            perm[s] = order[s];
+        }
+        blocks[b] = make_float4(cf[0], cf[1], cf[2], (float)(rad * (1 + 1e-6) + 1e-4));
+    }
+    for (int m = 0; m < nm; m++) {
+        double mx = 0;
+        for (int i = 0; i < n; i++) {
+            double d2 = 0;
+            for (int k = 0; k < 3; k++) {
+                const double x = msoa[((size_t)m * 3 + k) * n + i];
+                d2 += x * x;
+            }
+            mx = std::max(mx, std::sqrt(d2));
+        }
+        c->dev.lmode_max[m] = (float)(mx * (1 + 1e-5));
+    }
+    c->lblocks.upload(blocks);
+    c->lperm.upload(perm);
+    c->dev.nlblocks = nb;
+}
+
+extern "C" {
+
+const char *sdgso_last_error(void) { return g_error.c_str(); }
+
+int sdgso_version(void) { return 100; }
+
+void sdgso_default_params(sdgso_params *p) {
+    if (!p) return;
+    p->rho = 0.4;
+    p->gamma = 0.6;
+    p->beta = 0.08;
+    p->vision0 = 0.2;
+    p->max_vision = 5.0;
+    p->luciferin0 = 5.0;
+    p->max_neighbors = 5;
+    p->step_translation = 0.5;
+    p->step_rotation = 0.5;
+    p->step_nmodes = 0.5;
+}
+
+int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, const int *rec_mask,
+                         int n_rec_modes, const double *rec_modes, int n_lig, const double *lig_xyz,
+                         const int *lig_type, const int *lig_mask, int n_lig_modes,
+                         const double *lig_modes, const double *lig_ref, const float *table,
+                         sdgso_complex *out) {
+    return guarded([&] {
+        if (!out) fail(SDGSO_EINVAL, "out is null");
+        *out = nullptr;
+        if (n_rec <= 0 || n_lig <= 0) fail(SDGSO_EINVAL, "atom counts must be positive");
+        if (n_rec_modes < 0 || n_rec_modes > SDGSO_MAX_MODES || n_lig_modes < 0 ||
+            n_lig_modes > SDGSO_MAX_MODES)
+            fail(SDGSO_EINVAL, "mode counts must be in [0, 32]");
+        if (!rec_xyz || !lig_xyz || !rec_type || !lig_type || !lig_ref)
+            fail(SDGSO_EINVAL, "null coordinate/type/reference pointer");
+        if ((n_rec_modes && !rec_modes) || (n_lig_modes && !lig_modes))
+            fail(SDGSO_EINVAL, "modes pointer is null");
+        check_table_and_types(table, rec_type, n_rec, lig_type, n_lig);
+        ensure_device();
+        std::unique_ptr<sdgso_complex_s> c(new sdgso_complex_s);
+        c->nr = n_rec;
+        c->nl = n_lig;
+        c->nmr = n_rec_modes;
+        c->nml = n_lig_modes;
+        std::vector<double> rsoa, rmsoa, lsoa, lmsoa;
+        soa_inputs(n_rec, rec_xyz, rec_mask, n_rec_modes, rec_modes, rsoa, rmsoa);
+        soa_inputs(n_lig, lig_xyz, lig_mask, n_lig_modes, lig_modes, lsoa, lmsoa);
+        c->r0.upload(rsoa);
+        c->rmodes.upload(rmsoa);
+        c->l0.upload(lsoa);
+        c->lmodes.upload(lmsoa);
+        std::vector<int> rt(n_rec), lt(n_lig);
+        for (int i = 0; i < n_rec; i++) rt[i] = rec_type[i] * SDGSO_NTYPES * SDGSO_NBINS;
+        for (int i = 0; i < n_lig; i++) lt[i] = lig_type[i] * SDGSO_NBINS;
+        c->rtype.upload(rt);
+        c->ltype.upload(lt);
+        std::vector<float> tab(kTableLen, 0.0f);
+        std::memcpy(tab.data(), table, sizeof(float) * SDGSO_TABLE_LEN);
+        c->table.upload(tab);
+        DevComplex &d = c->dev;
+        d.nr = n_rec;
+        d.nl = n_lig;
+        d.nmr = n_rec_modes;
+        d.nml = n_lig_modes;
+        build_tiles(c.get(), rec_xyz, rec_type, rmsoa);
+        build_lblocks(c.get(), lig_xyz, lmsoa);
+        d.r0 = c->r0.p;
+        d.rmodes = c->rmodes.p;
+        d.l0 = c->l0.p;
+        d.lmodes = c->lmodes.p;
+        d.rtype = c->rtype.p;
+        d.ltype = c->ltype.p;
+        d.table = c->table.p;
+        for (int k = 0; k < 3; k++) d.lref[k] = lig_ref[k];
+        d.tiles = c->tiles.p;
+        d.rmod32 = c->rmod32.p;
+        d.rtype_s = c->rtype_s.p;
+        d.rperm = c->rperm.p;
+        d.lperm = c->lperm.p;
+        d.lblocks = c->lblocks.p;
+        d.rrel = c->rrel.p;
+        *out = c.release();
+    });
+}
+
+int sdgso_complex_destroy(sdgso_complex c) {
+    return guarded([&] {
+        if (!c) return;
+        if (g_device >= 0) (void)hipSetDevice(g_device);
+        delete c;
+    });
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- runs
+static void profile_pair(sdgso_run r, std::vector<std::pair<hipEvent_t, hipEvent_t>> &v,
+                         std::pair<hipEvent_t, hipEvent_t> &out) {
+    HIPCHK(hipEventCreate(&out.first));
+    HIPCHK(hipEventCreate(&out.second));
+    v.push_back(out);
+    (void)r;
+}
+
+static ScoreArgs score_args(sdgso_run r) {
+    ScoreArgs a{};
+    a.cx = r->dcx.p;
+    a.swarm_cx = r->swarm_cx.p;
+    a.G = r->G;
+    a.P = r->P;
+    a.n_glow = r->N;
+    a.pos = r->pos.p;
+    a.score = r->score.p;
+    a.luciferin = r->luciferin.p;
+    a.ref = r->ref.p;
+    a.pairs = r->pairs.p;
+    a.step = r->step.p;
+    a.rho = r->params.rho;
+    a.gamma = r->params.gamma;
+    a.update_luciferin = 1;
+    a.band = r->band;
+    a.guard_count = r->guard_count.p;
+    a.partial = r->partial.p;
+    a.partial_pairs = r->partial_pairs.p;
+    a.max_lblocks = r->max_lblocks;
+    a.work = r->work.p;
+    a.n_work = (int)r->work.n;
+    return a;
+}
+
+// The scoring kernel (score64 or score32) sits between the two optional
+// events; finish32 (a per-glowworm epilogue) runs after them.
+static void enqueue_score(sdgso_run r, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
+    ScoreArgs a = score_args(r);
+    if (ev0) HIPCHK(hipEventRecord(ev0, r->stream_h));
+    if (r->form == SDGSO_FP64) {
+        HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
+        if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
+    } else {
+        HIPCHK(launch_score32(a, r->stream_h));
+        if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
+        HIPCHK(launch_finish32(a, r->stream_h));
+    }
+}
+
+static void enqueue_select_move(sdgso_run r, int do_select, int do_move) {
+    SelectMoveArgs m{};
+    m.G = r->G;
+    m.P = r->P;
+    m.nmr = r->nmr;
+    m.nml = r->nml;
+    m.n_swarms = r->S;
+    m.pos = r->pos.p;
+    m.luciferin = r->luciferin.p;
+    m.ref = r->ref.p;
+    m.vision = r->vision.p;
+    m.nnei = r->nnei.p;
+    m.moved = r->moved.p;
+    m.select = r->select.p;
+    m.stream = r->stream.p;
+    m.n_steps = r->n_steps;
+    m.k = r->k;
+    m.d_k = r->d_k.p;
+    m.beta = r->params.beta;
+    m.max_vision = r->params.max_vision;
+    m.max_neighbors = r->params.max_neighbors;
+    m.step_t = r->params.step_translation;
+    m.step_r = r->params.step_rotation;
+    m.step_nm = r->params.step_nmodes;
+    m.do_select = do_select;
+    m.do_move = do_move;
+    HIPCHK(launch_select_move(m, r->stream_h));
+}
+
+static void enqueue_step(sdgso_run r, bool timed) {
+    std::pair<hipEvent_t, hipEvent_t> e1{}, e2{};
+    if (timed) {
+        profile_pair(r, r->ev_score, e1);
+        profile_pair(r, r->ev_move, e2);
+    }
+    enqueue_score(r, e1.first, e1.second);
+    if (timed) HIPCHK(hipEventRecord(e2.first, r->stream_h));
+    enqueue_select_move(r, 1, 1);
+    if (timed) HIPCHK(hipEventRecord(e2.second, r->stream_h));
+    HIPCHK(launch_advance(r->d_k.p, r->stream_h));
+}
+
+static void drop_graph(sdgso_run r) {
+    if (r->graph) (void)hipGraphExecDestroy(r->graph);
+    r->graph = nullptr;
+}
+
+static void harvest_profile(sdgso_run r) {
+    for (auto &e : r->ev_score) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        r->t_score += ms;
+        r->n_score++;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    for (auto &e : r->ev_move) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        r->t_move += ms;
+        r->n_move++;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
+    r->ev_score.clear();
+    r->ev_move.clear();
+}
+
+extern "C" {
+
+int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n_swarms,
+                     int n_glowworms, const double *pos, const double *stream, int n_steps,
+                     const sdgso_params *params, int form, double guard_band, sdgso_run *out) {
+    return guarded([&] {
+        if (!out) fail(SDGSO_EINVAL, "out is null");
+        *out = nullptr;
+        if (n_complex <= 0 || !complexes || !n_swarms) fail(SDGSO_EINVAL, "no complexes");
+        if (n_glowworms <= 0 || n_glowworms > 1024)
+            fail(SDGSO_EINVAL, "glowworms per swarm must be in [1, 1024]");
+        if (n_steps < 0 || (n_steps > 0 && !stream)) fail(SDGSO_EINVAL, "stream missing");
+        if (!pos) fail(SDGSO_EINVAL, "pos is null");
+        if (form != SDGSO_FP64 && form != SDGSO_FP32_GUARD) fail(SDGSO_EINVAL, "unknown form");
+        ensure_device();
+        std::unique_ptr<sdgso_run_s> r(new sdgso_run_s);
+        r->n_complex = n_complex;
+        r->G = n_glowworms;
+        r->form = form;
+        r->band = (float)(guard_band > 0 ? guard_band : 1e-4);
+        if (params)
+            r->params = *params;
+        else
+            sdgso_default_params(&r->params);
+        std::vector<DevComplex> descs;
+        std::vector<int> scx;
+        for (int k = 0; k < n_complex; k++) {
+            sdgso_complex c = complexes[k];
+            if (!c) fail(SDGSO_EINVAL, "null complex handle");
+            if (n_swarms[k] <= 0) fail(SDGSO_EINVAL, "swarm counts must be positive");
+            if (k == 0) {
+                r->nmr = c->nmr;
+                r->nml = c->nml;
+            } else if (c->nmr != r->nmr || c->nml != r->nml) {
+                fail(SDGSO_EINVAL, "all complexes of a run need the same mode counts");
+            }
+            r->cxs.push_back(c);
+            r->n_swarms.push_back(n_swarms[k]);
+            descs.push_back(c->dev);
+            for (int s = 0; s < n_swarms[k]; s++) scx.push_back(k);
+            r->max_nl = std::max(r->max_nl, c->nl);
+            r->max_lblocks = std::max(r->max_lblocks, c->dev.nlblocks);
+            r->S += n_swarms[k];
+        }
+        r->P = 7 + r->nmr + r->nml;
+        const size_t N = (size_t)r->S * r->G;
+        if (N > (size_t)1 << 30) fail(SDGSO_EINVAL, "too many glowworms");
+        r->N = (int)N;
+        r->n_steps = n_steps;
+        // LDS limits of the kernels
+        const size_t lds64 = sizeof(double) * (3 * (size_t)r->max_nl + 512) + sizeof(int) * r->max_nl;
+        const size_t ldsg = sizeof(double) * (4 * (size_t)r->G + (size_t)((r->G + 63) / 64 * 64) * r->P);
+        if (form == SDGSO_FP64 && lds64 > 160 * 1024)
+            fail(SDGSO_EINVAL, "ligand too large for the FP64 form (" + std::to_string(r->max_nl) + " atoms)");
+        if (ldsg > 160 * 1024) fail(SDGSO_EINVAL, "glowworms x pose length exceed the LDS of one workgroup");
+        r->dcx.upload(descs);
+        r->swarm_cx.upload(scx);
+        r->pos.upload(pos, N * r->P);
+        std::vector<double> init((size_t)N);
+        std::fill(init.begin(), init.end(), r->params.luciferin0);
+        r->luciferin.upload(init);
+        std::fill(init.begin(), init.end(), 0.0);
+        r->score.upload(init);
+        std::fill(init.begin(), init.end(), r->params.vision0);
+        r->vision.upload(init);
+        std::vector<double> zeros3(3 * N, 0.0);
+        r->ref.upload(zeros3);
+        std::vector<int> zi(N, 0);
+        r->nnei.upload(zi);
+        r->moved.upload(zi);
+        r->step.upload(zi);
+        for (size_t i = 0; i < N; i++) zi[i] = (int)(i % r->G);
+        r->select.upload(zi);
+        std::vector<int64_t> z64(N, 0);
+        r->pairs.upload(z64);
+        if (n_steps > 0) r->stream.upload(stream, (size_t)r->S * n_steps * r->G);
+        std::vector<unsigned long long> zc(4, 0);
+        r->guard_count.upload(zc);
+        std::vector<int> k0(1, 0);
+        r->d_k.upload(k0);
+        if (form == SDGSO_FP32_GUARD) {
+            r->partial.alloc(N * (size_t)r->max_lblocks);
+            r->partial_pairs.alloc(N * (size_t)r->max_lblocks);
+            // work list: (glowworm, ligand block) pairs, blocks of one glowworm adjacent
+            std::vector<int> work;
+            size_t g0 = 0;
+            for (int k = 0; k < n_complex; k++) {
+                const int nb = r->cxs[k]->dev.nlblocks;
+                const size_t ng = (size_t)n_swarms[k] * r->G;
+                for (size_t g = 0; g < ng; g++)
+                    for (int b = 0; b < nb; b++) {
+                        work.push_back((int)(g0 + g));
+                        work.push_back(b);
+                    }
+                g0 += ng;
+            }
+            r->work.upload(work);
+            r->work.n = work.size() / 2;
+        }
+        HIPCHK(hipStreamCreateWithFlags(&r->own_stream, hipStreamNonBlocking));
+        r->stream_h = r->own_stream;
+        *out = r.release();
+    });
+}
+
+int sdgso_run_destroy(sdgso_run r) {
+    return guarded([&] {
+        if (!r) return;
+        if (g_device >= 0) (void)hipSetDevice(g_device);
+        if (r->stream_h) (void)hipStreamSynchronize(r->stream_h);
+        drop_graph(r);
+        for (auto &e : r->ev_score) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        for (auto &e : r->ev_move) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
+        if (r->own_stream) (void)hipStreamDestroy(r->own_stream);
+        delete r;
+    });
+}
+
+int sdgso_run_set_stream(sdgso_run r, void *hip_stream) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        drop_graph(r);
+        r->stream_h = hip_stream ? (hipStream_t)hip_stream : r->own_stream;
+    });
+}
+
+int sdgso_run_steps(sdgso_run r, int n, int use_graph) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (n < 0) fail(SDGSO_EINVAL, "negative step count");
+        if (r->k + n > r->n_steps)
+            fail(SDGSO_ESTATE, "stepping past the random stream (" + std::to_string(r->n_steps) +
+                                   " steps provided)");
+        ensure_device();
+        if (use_graph && !r->profiling && n > 0) {
+            // One step captured once; the stream index lives on the device
+            // (advanced by the last node), so every replay is the same graph.
+            if (!r->graph) {
+                hipGraph_t g = nullptr;
+                HIPCHK(hipStreamBeginCapture(r->stream_h, hipStreamCaptureModeThreadLocal));
+                enqueue_step(r, false);
+                HIPCHK(hipStreamEndCapture(r->stream_h, &g));
+                const hipError_t e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(g);
+                if (e != hipSuccess) {
+                    r->graph = nullptr;
+                    fail(SDGSO_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+                }
+            }
+            for (int i = 0; i < n; i++) {
+                HIPCHK(hipGraphLaunch(r->graph, r->stream_h));
+                r->k++;
+            }
+            return;
+        }
+        for (int i = 0; i < n; i++) {
+            enqueue_step(r, r->profiling);
+            r->k++;
+        }
+    });
+}
+
+int sdgso_run_phases(sdgso_run r, int mask) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        ensure_device();
+        if ((mask & 2) && r->k >= r->n_steps) fail(SDGSO_ESTATE, "no stream left for selection");
+        if (mask & 1) enqueue_score(r);
+        if (mask & 6) enqueue_select_move(r, (mask & 2) ? 1 : 0, (mask & 4) ? 1 : 0);
+        if (mask & 4) {
+            HIPCHK(launch_advance(r->d_k.p, r->stream_h));
+            r->k++;
+        }
+    });
+}
+
+int sdgso_run_sync(sdgso_run r) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+    });
+}
+
+int sdgso_run_step_index(sdgso_run r) { return r ? r->k : -1; }
+
+int sdgso_run_read(sdgso_run r, double *pos, double *luciferin, double *score, double *vision,
+                   int *nnei, int *moved, int *select, double *ref, int64_t *pairs) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        const size_t N = r->N;
+        auto get = [&](void *dst, const void *src, size_t bytes) {
+            if (dst) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        };
+        get(pos, r->pos.p, N * r->P * sizeof(double));
+        get(luciferin, r->luciferin.p, N * sizeof(double));
+        get(score, r->score.p, N * sizeof(double));
+        get(vision, r->vision.p, N * sizeof(double));
+        get(nnei, r->nnei.p, N * sizeof(int));
+        get(moved, r->moved.p, N * sizeof(int));
+        get(select, r->select.p, N * sizeof(int));
+        get(ref, r->ref.p, 3 * N * sizeof(double));
+        get(pairs, r->pairs.p, N * sizeof(int64_t));
+    });
+}
+
+int sdgso_run_write(sdgso_run r, const double *pos, const double *luciferin, const double *score,
+                    const double *vision, const int *moved, const double *ref) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        const size_t N = r->N;
+        auto put = [&](void *dst, const void *src, size_t bytes) {
+            if (src) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        };
+        put(r->pos.p, pos, N * r->P * sizeof(double));
+        put(r->luciferin.p, luciferin, N * sizeof(double));
+        put(r->score.p, score, N * sizeof(double));
+        put(r->vision.p, vision, N * sizeof(double));
+        put(r->moved.p, moved, N * sizeof(int));
+        put(r->ref.p, ref, 3 * N * sizeof(double));
+    });
+}
+
+int sdgso_run_profile(sdgso_run r, int enable) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        harvest_profile(r);
+        r->t_score = r->t_move = 0;
+        r->n_score = r->n_move = 0;
+        r->profiling = enable != 0;
+        HIPCHK(hipMemset(r->guard_count.p, 0, sizeof(unsigned long long) * 4));
+    });
+}
+
+int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        harvest_profile(r);
+        if (times_ms) {
+            times_ms[0] = r->t_score;
+            times_ms[1] = r->t_move;
+        }
+        if (launches) {
+            launches[0] = r->n_score;
+            launches[1] = r->n_move;
+        }
+    });
+}
+
+int sdgso_run_guard_stats(sdgso_run r, int64_t *stats) {
+    return guarded([&] {
+        if (!r || !stats) fail(SDGSO_EINVAL, "null argument");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        unsigned long long h[4];
+        HIPCHK(hipMemcpy(h, r->guard_count.p, sizeof(h), hipMemcpyDeviceToHost));
+        stats[0] = (int64_t)h[0];
+    });
+}
+
+// ---------------------------------------------------------------- scoring plugin
+int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int form,
+                      double guard_band, double *out_score, double *out_ref, int64_t *out_pairs) {
+    return guarded([&] {
+        if (!c) fail(SDGSO_EINVAL, "null complex");
+        if (n_poses < 0 || (n_poses > 0 && (!poses || !out_score)))
+            fail(SDGSO_EINVAL, "bad pose buffers");
+        if (n_poses == 0) return;
+        // a run of n_poses single-glowworm swarms, scored once without luciferin
+        sdgso_run r = nullptr;
+        sdgso_complex cs[1] = {c};
+        int ns[1] = {n_poses};
+        int rc = sdgso_run_create(1, cs, ns, 1, poses, nullptr, 0, nullptr, form, guard_band, &r);
+        if (rc != SDGSO_OK) throw Failure{rc};
+        std::unique_ptr<sdgso_run_s, int (*)(sdgso_run)> hold(r, sdgso_run_destroy);
+        ScoreArgs a = score_args(r);
+        a.update_luciferin = 0;
+        if (form == SDGSO_FP64) {
+            HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
+        } else {
+            HIPCHK(launch_score32(a, r->stream_h));
+            HIPCHK(launch_finish32(a, r->stream_h));
+        }
+        rc = sdgso_run_read(r, nullptr, nullptr, out_score, nullptr, nullptr, nullptr, nullptr,
+                            out_ref, out_pairs);
+        if (rc != SDGSO_OK) throw Failure{rc};
+    });
+}
+
+// ---------------------------------------------------------------- host utilities
+int sdgso_stream(uint32_t seed, int n_swarms, int n_steps, int n_glowworms, double *u) {
+    return guarded([&] {
+        if (n_swarms < 0 || n_steps < 0 || n_glowworms < 0) fail(SDGSO_EINVAL, "negative size");
+        if (!u && (size_t)n_swarms * n_steps * n_glowworms) fail(SDGSO_EINVAL, "u is null");
+        // prepare_gso_tasks.cc:409-427: one engine, drawn serially, swarm-major
+        std::mt19937 gen(seed);
+        std::uniform_real_distribution<double> dist(0.0, 1.0);
+        const size_t n = (size_t)n_swarms * n_steps * n_glowworms;
+        for (size_t i = 0; i < n; i++) u[i] = dist(gen);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,106 @@
+// Device helpers that restate the reference's FP64 pose arithmetic in its
+// operation order.  Every translation unit that includes this file is built
+// with -ffp-contract=off, so no multiply-add here is fused.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "sd_internal.h"
+
+namespace sd {
+
+// inverse = conj(q) / |q|^2 (prepare_gso_tasks.cc:288-302)
+__device__ __forceinline__ void quat_inverse(const double q[4], double inv[4]) {
+    const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
+    inv[0] = q[0] / n2;
+    inv[1] = -q[1] / n2;
+    inv[2] = -q[2] / n2;
+    inv[3] = -q[3] / n2;
+}
+
+// q * (0, p) * inv, keeping the zero terms and the left-to-right order of
+// prepare_gso_tasks.cc:304-329.
+__device__ __forceinline__ void rotate_point(const double q[4], const double inv[4], double &x,
+                                             double &y, double &z) {
+    const double a0 = 0.0, a1 = x, a2 = y, a3 = z;
+    const double b0 = q[0] * a0 - q[1] * a1 - q[2] * a2 - q[3] * a3;
+    const double b1 = q[0] * a1 + q[1] * a0 + q[2] * a3 - q[3] * a2;
+    const double b2 = q[0] * a2 - q[1] * a3 + q[2] * a0 + q[3] * a1;
+    const double b3 = q[0] * a3 + q[1] * a2 - q[2] * a1 + q[3] * a0;
+    x = b0 * inv[1] + b1 * inv[0] + b2 * inv[3] - b3 * inv[2];
+    y = b0 * inv[2] - b1 * inv[3] + b2 * inv[0] + b3 * inv[1];
+    z = b0 * inv[3] + b1 * inv[2] - b2 * inv[1] + b3 * inv[0];
+}
+
+// dist_to_bins[d] - 1 (interface.h:30-35) for d in [0, 29]:
+// 0,0,0,1,...,13 (d = 3..15), then pairs 13,14,14,...,19,19, then 20 at d = 29.
+__device__ __forceinline__ int bin_of_d(int d) { return max(0, min(d - 2, (d + 11) >> 1)); }
+
+// Receptor atom i displaced by the normal modes, FP64, reference order
+// (prepare_gso_tasks.cc:465-482; masked atoms have zero mode rows).
+__device__ __forceinline__ void receptor_atom64(const DevComplex &c, const double *ext, int i,
+                                                double &x, double &y, double &z) {
+    x = c.r0[i];
+    y = c.r0[c.nr + i];
+    z = c.r0[2 * c.nr + i];
+    for (int m = 0; m < c.nmr; m++) {
+        const double e = ext[m];
+        const double *M = c.rmodes + (size_t)m * 3 * c.nr;
+        x += M[i] * e;
+        y += M[c.nr + i] * e;
+        z += M[2 * c.nr + i] * e;
+    }
+}
+
+// Ligand atom j of a pose: modes, rotation, translation
+// (prepare_gso_tasks.cc:487-526).
+__device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double *pose,
+                                              const double inv[4], int j, double &x, double &y,
+                                              double &z) {
+    x = c.l0[j];
+    y = c.l0[c.nl + j];
+    z = c.l0[2 * c.nl + j];
+    const double *ext = pose + 7 + c.nmr;
+    for (int m = 0; m < c.nml; m++) {
+        const double e = ext[m];
+        const double *M = c.lmodes + (size_t)m * 3 * c.nl;
+        x += M[j] * e;
+        y += M[c.nl + j] * e;
+        z += M[2 * c.nl + j] * e;
+    }
+    rotate_point(pose + 3, inv, x, y, z);
+    x += pose[0];
+    y += pose[1];
+    z += pose[2];
+}
+
+// Ligand reference point of a pose (rotation + translation, no modes).
+__device__ __forceinline__ void ref_point64(const DevComplex &c, const double *pose,
+                                            const double inv[4], double out[3]) {
+    double x = c.lref[0], y = c.lref[1], z = c.lref[2];
+    rotate_point(pose + 3, inv, x, y, z);
+    out[0] = x + pose[0];
+    out[1] = y + pose[1];
+    out[2] = z + pose[2];
+}
+
+// DFIRE contribution of one pair in FP64 (prepare_gso_tasks.cc:1542-1566);
+// returns false when the pair is outside (0, 225].
+__device__ __forceinline__ bool pair_term64(double rx, double ry, double rz, double lx, double ly,
+                                            double lz, int ra, int lb, const float *table,
+                                            double &val) {
+    const double dx = rx - lx;
+    const double dy = ry - ly;
+    const double dz = rz - lz;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 <= kCutoff2 && d2 > 0.0) {
+        const int d = (int)(sqrt(d2) * 2.0 - 1.0);
+        val = (double)table[ra + lb + bin_of_d(d)];
+        return true;
+    }
+    return false;
+}
+
+__device__ __forceinline__ double score_of_energy(double e) { return (e * 0.0157 - 4.7) * -1; }
+
+}  // namespace sd

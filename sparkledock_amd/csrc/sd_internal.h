@@ -1,0 +1,121 @@
+// Internal declarations shared by the HIP kernels and the C-ABI layer of
+// libsdgso.so.  Device-side data layouts (all in HBM, owned by the library):
+//
+//   complex (one per docking problem)
+//     r0      f64 [3][nr]          receptor coordinates, SoA
+//     rmodes  f64 [nmr][3][nr]     receptor modes, SoA, mask folded in (zeros)
+//     l0      f64 [3][nl]          ligand coordinates, SoA
+//     lmodes  f64 [nml][3][nl]
+//     rtype   i32 [nr]             type * 3360 (row offset into the table)
+//     ltype   i32 [nl]             type * 20
+//     table   f32 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
+//   receptor tiles (FP32 guard form): atoms in Morton order, 64 per tile
+//     tiles   f32x4 [ntiles]       tile centre (x,y,z) and radius
+//     rrel    f64 [3][ntiles*64]   coordinates - tile centre
+//     rmod32  f32 [nmr][3][ntiles*64]
+//     rtype_s i32 [ntiles*64]      type*3360 in tile order
+//     rperm   i32 [ntiles*64]      original atom index (-1 = padding)
+//   ligand blocks (FP32 guard form): atoms in Morton order, 64 per block
+//     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
+//     lblocks f32x4 [nlblocks]     block centre in the ligand frame, radius
+//   run state, N = sum(swarms) * G glowworms
+//     pos [N][P] f64, luciferin/score/vision [N] f64, ref [N][3] f64,
+//     nnei/moved/select/step [N] i32, pairs [N] i64,
+//     stream [sum S][n_steps][G] f64
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "../../include/sdgso.h"
+
+namespace sd {
+
+constexpr int kTableLen = SDGSO_TABLE_LEN + SDGSO_NBINS;
+constexpr int kTile = 64;              // receptor atoms per tile = wavefront width
+constexpr int kBlock = 64;             // ligand atoms per block = wavefront width
+constexpr double kQuantScale = 1048576.0;  // 2^20: fixed-point units per A
+constexpr double kCutoff2 = 225.0;
+
+// Per-complex device descriptor (plain data; lives in a device array).
+struct DevComplex {
+    int nr, nl, nmr, nml;
+    const double *r0;
+    const double *rmodes;
+    const double *l0;
+    const double *lmodes;
+    const int *rtype;
+    const int *ltype;
+    const float *table;
+    double lref[3];
+    // receptor tiles
+    int ntiles;
+    const float4 *tiles;
+    const double *rrel;      // [3][ntiles*64] coordinates - tile centre (f64)
+    const float *rmod32;
+    const int *rtype_s;
+    const int *rperm;
+    float rmode_max[SDGSO_MAX_MODES];  // max over atoms of |M_m,i|
+    // ligand blocks
+    int nlblocks;
+    const int *lperm;
+    const float4 *lblocks;
+    float lmode_max[SDGSO_MAX_MODES];
+};
+
+// Arguments of one scoring launch over a set of glowworms.
+struct ScoreArgs {
+    const DevComplex *cx;   // [n_complex]
+    const int *swarm_cx;    // [n_swarms] complex of each swarm
+    int G, P;
+    int n_glow;             // glowworms in this launch
+    const double *pos;      // [n_glow][P]
+    double *score;          // [n_glow]
+    double *luciferin;      // [n_glow] (updated when update_luciferin)
+    double *ref;            // [n_glow][3]
+    int64_t *pairs;         // [n_glow] or null
+    int *step;              // [n_glow] or null
+    double rho, gamma;
+    int update_luciferin;
+    // FP32 guard form
+    float band;             // guard band (A^2)
+    unsigned long long *guard_count;  // FP64 recomputes (optional)
+    double *partial;        // [n_glow][max_lblocks] per-block energies (FP32 form)
+    long long *partial_pairs;  // [n_glow][max_lblocks] per-block pair counts
+    int max_lblocks;
+    const int *work;        // FP32 form: work list of (glowworm, ligand block), or null
+    int n_work;
+};
+
+struct SelectMoveArgs {
+    int G, P, nmr, nml;
+    int n_swarms;
+    double *pos;            // [N][P]
+    const double *luciferin;
+    const double *ref;
+    double *vision;
+    int *nnei;
+    int *moved;
+    int *select;
+    const double *stream;   // [n_swarms][n_steps][G]
+    int n_steps;
+    int k;                  // step index into the stream (when d_k is null)
+    int *d_k;               // device-resident step index (graph replay)
+    double beta, max_vision;
+    int max_neighbors;
+    double step_t, step_r, step_nm;
+    int do_select, do_move;
+};
+
+// Launchers (defined in the .hip files).
+hipError_t launch_score64(const ScoreArgs &a, int max_nl, hipStream_t s);
+hipError_t launch_score32(const ScoreArgs &a, hipStream_t s);
+hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s);
+hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s);
+hipError_t launch_advance(int *d_k, hipStream_t s);
+
+void set_error(const std::string &msg);
+
+}  // namespace sd

@@ -1,0 +1,278 @@
+"""ctypes binding of libsdgso.so (include/sdgso.h).
+
+The library is the product path: every call here goes to the HIP kernels.  If
+libsdgso.so is missing, `lib()` raises; there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsdgso.so")
+
+FP64 = 0
+FP32_GUARD = 1
+NTYPES = 168
+NBINS = 20
+TABLE_LEN = NTYPES * NTYPES * NBINS
+
+_D = ctypes.POINTER(ctypes.c_double)
+_F = ctypes.POINTER(ctypes.c_float)
+_I = ctypes.POINTER(ctypes.c_int)
+_I64 = ctypes.POINTER(ctypes.c_int64)
+_VP = ctypes.c_void_p
+
+
+class Params(ctypes.Structure):
+    _fields_ = [
+        ("rho", ctypes.c_double), ("gamma", ctypes.c_double), ("beta", ctypes.c_double),
+        ("vision0", ctypes.c_double), ("max_vision", ctypes.c_double),
+        ("luciferin0", ctypes.c_double), ("max_neighbors", ctypes.c_int),
+        ("step_translation", ctypes.c_double), ("step_rotation", ctypes.c_double),
+        ("step_nmodes", ctypes.c_double),
+    ]
+
+
+class SdgsoError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("sdgso error %d: %s" % (code, msg))
+        self.code = code
+
+
+# (name, restype, argtypes): every symbol include/sdgso.h declares
+SIGNATURES = [
+    ("sdgso_last_error", ctypes.c_char_p, []),
+    ("sdgso_version", ctypes.c_int, []),
+    ("sdgso_default_params", None, [ctypes.POINTER(Params)]),
+    ("sdgso_complex_create", ctypes.c_int,
+     [ctypes.c_int, _D, _I, _I, ctypes.c_int, _D, ctypes.c_int, _D, _I, _I, ctypes.c_int, _D, _D,
+      _F, ctypes.POINTER(_VP)]),
+    ("sdgso_complex_destroy", ctypes.c_int, [_VP]),
+    ("sdgso_score_poses", ctypes.c_int,
+     [_VP, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double, _D, _D, _I64]),
+    ("sdgso_run_create", ctypes.c_int,
+     [ctypes.c_int, ctypes.POINTER(_VP), _I, ctypes.c_int, _D, _D, ctypes.c_int,
+      ctypes.POINTER(Params), ctypes.c_int, ctypes.c_double, ctypes.POINTER(_VP)]),
+    ("sdgso_run_destroy", ctypes.c_int, [_VP]),
+    ("sdgso_run_set_stream", ctypes.c_int, [_VP, _VP]),
+    ("sdgso_run_steps", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int]),
+    ("sdgso_run_phases", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("sdgso_run_sync", ctypes.c_int, [_VP]),
+    ("sdgso_run_step_index", ctypes.c_int, [_VP]),
+    ("sdgso_run_read", ctypes.c_int, [_VP, _D, _D, _D, _D, _I, _I, _I, _D, _I64]),
+    ("sdgso_run_write", ctypes.c_int, [_VP, _D, _D, _D, _D, _I, _D]),
+    ("sdgso_run_profile", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("sdgso_run_profile_read", ctypes.c_int, [_VP, _D, _I64]),
+    ("sdgso_run_guard_stats", ctypes.c_int, [_VP, _I64]),
+    ("sdgso_stream", ctypes.c_int,
+     [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
+    ("sdgso_synth_molecule", ctypes.c_int,
+     [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _D, _I, _D]),
+    ("sdgso_synth_table", ctypes.c_int, [ctypes.c_uint64, _F]),
+    ("sdgso_synth_poses", ctypes.c_int,
+     [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+      ctypes.c_double, _D]),
+]
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsdgso.so is not built: run `python -m sparkledock_amd.build` "
+                               "(there is no CPU fallback)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise SdgsoError(rc, lib().sdgso_last_error().decode(errors="replace"))
+
+
+def _p(a, t=_D):
+    return None if a is None else a.ctypes.data_as(t)
+
+
+def default_params(**over):
+    p = Params()
+    lib().sdgso_default_params(ctypes.byref(p))
+    for k, v in over.items():
+        setattr(p, k, v)
+    return p
+
+
+class Complex:
+    """One docking problem resident on the device."""
+
+    def __init__(self, r_xyz, r_type, l_xyz, l_type, table, lig_ref, r_modes=None, l_modes=None,
+                 r_mask=None, l_mask=None):
+        r_xyz = np.ascontiguousarray(r_xyz, dtype=np.float64).reshape(-1, 3)
+        l_xyz = np.ascontiguousarray(l_xyz, dtype=np.float64).reshape(-1, 3)
+        self.nr, self.nl = len(r_xyz), len(l_xyz)
+        r_type = np.ascontiguousarray(r_type, dtype=np.int32)
+        l_type = np.ascontiguousarray(l_type, dtype=np.int32)
+        r_mask = None if r_mask is None else np.ascontiguousarray(r_mask, dtype=np.int32)
+        l_mask = None if l_mask is None else np.ascontiguousarray(l_mask, dtype=np.int32)
+        self.nmr = 0 if r_modes is None else int(np.asarray(r_modes).shape[0])
+        self.nml = 0 if l_modes is None else int(np.asarray(l_modes).shape[0])
+        rm = None if not self.nmr else np.ascontiguousarray(r_modes, dtype=np.float64)
+        lm = None if not self.nml else np.ascontiguousarray(l_modes, dtype=np.float64)
+        ref = np.ascontiguousarray(lig_ref, dtype=np.float64).reshape(3)
+        tab = np.ascontiguousarray(table, dtype=np.float32).reshape(-1)
+        if tab.size != TABLE_LEN:
+            raise ValueError("table must have 168*168*20 entries")
+        self.P = 7 + self.nmr + self.nml
+        h = _VP()
+        _check(lib().sdgso_complex_create(
+            self.nr, _p(r_xyz), _p(r_type, _I), _p(r_mask, _I), self.nmr, _p(rm),
+            self.nl, _p(l_xyz), _p(l_type, _I), _p(l_mask, _I), self.nml, _p(lm),
+            _p(ref), _p(tab, _F), ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sdgso_complex_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def score(self, poses, form=FP64, band=1e-4):
+        poses = np.ascontiguousarray(poses, dtype=np.float64).reshape(-1, self.P)
+        n = len(poses)
+        s = np.zeros(n)
+        ref = np.zeros((n, 3))
+        pairs = np.zeros(n, np.int64)
+        _check(lib().sdgso_score_poses(self.h, n, _p(poses), form, band, _p(s), _p(ref),
+                                       _p(pairs, _I64)))
+        return s, ref, pairs
+
+
+class Run:
+    """A GSO run over one or more complexes, state resident on the device."""
+
+    def __init__(self, complexes, n_swarms, G, pos, stream, n_steps, params=None, form=FP64,
+                 band=1e-4):
+        if isinstance(complexes, Complex):
+            complexes = [complexes]
+            n_swarms = [n_swarms]
+        self.complexes = list(complexes)
+        self.n_swarms = [int(s) for s in n_swarms]
+        self.S = sum(self.n_swarms)
+        self.G = int(G)
+        self.P = self.complexes[0].P
+        self.N = self.S * self.G
+        pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+        if pos.size != self.N * self.P:
+            raise ValueError("pos has %d values, expected %d" % (pos.size, self.N * self.P))
+        st = None if stream is None else np.ascontiguousarray(stream, dtype=np.float64).reshape(-1)
+        if st is not None and st.size != self.S * n_steps * self.G:
+            raise ValueError("stream has %d values, expected %d" % (st.size, self.S * n_steps * self.G))
+        hs = (_VP * len(self.complexes))(*[c.h for c in self.complexes])
+        ns = np.ascontiguousarray(self.n_swarms, dtype=np.int32)
+        prm = params or default_params()
+        h = _VP()
+        _check(lib().sdgso_run_create(len(self.complexes), hs, _p(ns, _I), self.G, _p(pos), _p(st),
+                                      int(n_steps), ctypes.byref(prm), form, band, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sdgso_run_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, hip_stream_ptr):
+        _check(lib().sdgso_run_set_stream(self.h, hip_stream_ptr))
+
+    def steps(self, n, graph=False):
+        _check(lib().sdgso_run_steps(self.h, int(n), 1 if graph else 0))
+
+    def phases(self, mask):
+        _check(lib().sdgso_run_phases(self.h, int(mask)))
+
+    def sync(self):
+        _check(lib().sdgso_run_sync(self.h))
+
+    @property
+    def k(self):
+        return lib().sdgso_run_step_index(self.h)
+
+    def read(self):
+        N, P = self.N, self.P
+        out = dict(pos=np.zeros((N, P)), luciferin=np.zeros(N), score=np.zeros(N),
+                   vision=np.zeros(N), nnei=np.zeros(N, np.int32), moved=np.zeros(N, np.int32),
+                   select=np.zeros(N, np.int32), ref=np.zeros((N, 3)),
+                   pairs=np.zeros(N, np.int64))
+        _check(lib().sdgso_run_read(self.h, _p(out["pos"]), _p(out["luciferin"]), _p(out["score"]),
+                                    _p(out["vision"]), _p(out["nnei"], _I), _p(out["moved"], _I),
+                                    _p(out["select"], _I), _p(out["ref"]), _p(out["pairs"], _I64)))
+        return out
+
+    def write(self, pos=None, luciferin=None, score=None, vision=None, moved=None, ref=None):
+        def arr(x, dt):
+            return None if x is None else np.ascontiguousarray(x, dtype=dt).reshape(-1)
+        pos, luciferin, score, vision, ref = (arr(pos, np.float64), arr(luciferin, np.float64),
+                                              arr(score, np.float64), arr(vision, np.float64),
+                                              arr(ref, np.float64))
+        moved = arr(moved, np.int32)
+        _check(lib().sdgso_run_write(self.h, _p(pos), _p(luciferin), _p(score), _p(vision),
+                                     _p(moved, _I), _p(ref)))
+
+    def profile(self, enable=True):
+        _check(lib().sdgso_run_profile(self.h, 1 if enable else 0))
+
+    def profile_read(self):
+        t = np.zeros(2)
+        n = np.zeros(2, np.int64)
+        _check(lib().sdgso_run_profile_read(self.h, _p(t), _p(n, _I64)))
+        return dict(score_ms=t[0], move_ms=t[1], score_launches=int(n[0]), move_launches=int(n[1]))
+
+    def guard_stats(self):
+        s = np.zeros(1, np.int64)
+        _check(lib().sdgso_run_guard_stats(self.h, _p(s, _I64)))
+        return dict(recomputed=int(s[0]))
+
+
+def stream(seed, S, n_steps, G):
+    u = np.zeros(S * n_steps * G)
+    _check(lib().sdgso_stream(seed, S, n_steps, G, _p(u)))
+    return u.reshape(S, n_steps, G)
+
+
+def synth_molecule(seed, n_atoms, n_modes):
+    xyz = np.zeros((n_atoms, 3))
+    t = np.zeros(n_atoms, np.int32)
+    m = np.zeros((n_modes, n_atoms, 3)) if n_modes else None
+    _check(lib().sdgso_synth_molecule(seed, n_atoms, n_modes, _p(xyz), _p(t, _I), _p(m)))
+    return xyz, t, m
+
+
+def synth_table(seed):
+    t = np.zeros(TABLE_LEN, np.float32)
+    _check(lib().sdgso_synth_table(seed, _p(t, _F)))
+    return t
+
+
+def synth_poses(seed, S, G, nmr, nml, centre_radius, swarm_radius=10.0):
+    pos = np.zeros((S, G, 7 + nmr + nml))
+    _check(lib().sdgso_synth_poses(seed, S, G, nmr, nml, float(centre_radius), float(swarm_radius),
+                                   _p(pos)))
+    return pos

@@ -1,0 +1,215 @@
+"""HIP kernels against the CPU oracle, through the C-ABI.
+
+Bars: FP64 form bit-exact (scores, pair counts, reference points); FP32 guard
+form with identical DFIRE bins (per-bin pair counts equal) and energy within
+1e-6 of the oracle relative to the sum of |T| over in-cutoff pairs; GSO
+choices (neighbour counts, selections, moved flags) bit-exact from identical
+states; poses equal up to the slerp's acos/sin (GPU libm vs glibc): 1e-14.
+"""
+import numpy as np
+import pytest
+
+import gpu_cases as GC
+import oracle_lib as O
+from sparkledock_amd import sdgso
+
+pytestmark = pytest.mark.gpu
+
+REL_E = 1e-6   # FP32 guard energy, relative to sum |T| over in-cutoff pairs
+POS_TOL = 1e-13
+
+
+def poses_of(case):
+    return case["pos"].reshape(-1, case["pos"].shape[-1])
+
+
+@pytest.mark.parametrize("seed,nmr,nml", [(1, 10, 10), (2, 0, 0), (3, 4, 7)])
+def test_score_fp64_bit_exact(seed, nmr, nml):
+    c = GC.case(seed, 500, 260, nmr, nml, 4, 16)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s, ref, pairs = pr.gpu.score(poses, sdgso.FP64)
+    s0, ref0, pairs0 = pr.oracle_scores(poses)
+    assert pairs0.min() > 0
+    assert np.array_equal(pairs, pairs0)
+    assert np.array_equal(ref, ref0)
+    assert np.array_equal(s, s0)
+
+
+@pytest.mark.parametrize("seed,nmr,nml", [(4, 10, 10), (5, 0, 0), (6, 20, 20)])
+def test_score_fp32_guard(seed, nmr, nml):
+    c = GC.case(seed, 700, 300, nmr, nml, 4, 16)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s, ref, pairs = pr.gpu.score(poses, sdgso.FP32_GUARD)
+    s0, ref0, pairs0 = pr.oracle_scores(poses)
+    assert np.array_equal(pairs, pairs0)
+    assert np.array_equal(ref, ref0)
+    e, e0 = (4.7 - s) / 0.0157, (4.7 - s0) / 0.0157
+    denom = GC.sum_abs_terms(pr, poses)
+    assert np.all(np.abs(e - e0) <= REL_E * np.maximum(denom, 1.0))
+
+
+def per_bin_counts(pr, poses, form):
+    """Pair count of every bin through one-hot tables (exact integer sums)."""
+    out = []
+    for b in range(20):
+        t = np.zeros((168, 168, 20), np.float32)
+        t[:, :, b] = 1.0
+        cx = pr.gpu_with_table(t.reshape(-1))
+        s, _, _ = cx.score(poses, form)
+        out.append(np.rint((4.7 - s) / 0.0157))
+        cx.close()
+    return np.array(out)
+
+
+def test_bins_exact_on_lattice_boundaries():
+    """Lattice geometry puts many pairs exactly on k^2/4 boundaries, on the
+    225 A^2 cutoff and at d2 = 0: both forms must bin them as the oracle."""
+    c = GC.grid_case(7)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    want = []
+    for b in range(20):
+        t = np.zeros((168, 168, 20), np.float32)
+        t[:, :, b] = 1.0
+        cx = pr.with_table(t.reshape(-1))
+        want.append([round((4.7 - cx.score_pose(q)[0]) / 0.0157) for q in poses])
+    want = np.array(want)
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        got = per_bin_counts(pr, poses, form)
+        assert np.array_equal(got, want), form
+    # the full table too (FP64: bit-exact; FP32: same pairs)
+    s0, _, p0 = pr.oracle_scores(poses)
+    s, _, p = pr.gpu.score(poses, sdgso.FP64)
+    assert np.array_equal(s, s0) and np.array_equal(p, p0)
+    s, _, p = pr.gpu.score(poses, sdgso.FP32_GUARD)
+    assert np.array_equal(p, p0)
+
+
+def test_bins_exact_random_fp32():
+    c = GC.case(8, 900, 400, 10, 10, 3, 8)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    a = per_bin_counts(pr, poses, sdgso.FP64)
+    b = per_bin_counts(pr, poses, sdgso.FP32_GUARD)
+    assert np.array_equal(a, b)
+    assert a.sum() > 10000
+
+
+def make_run(pr, c, steps, form=sdgso.FP64, stream=None):
+    S, G = c["pos"].shape[:2]
+    u = O.stream(0, S, steps, G) if stream is None else stream
+    return sdgso.Run(pr.gpu, S, G, c["pos"], u, steps, form=form), u
+
+
+def test_single_step_choices_from_identical_state():
+    """Neighbour lists, roulette choices and moves from one identical state."""
+    c = GC.case(9, 300, 150, 10, 10, 4, 64, swarm_radius=2.0)
+    pr = GC.Pair(c)
+    S, G = 4, 64
+    steps = 8
+    u = O.stream(0, S, steps, G)
+    prm = O.default_params()
+    st = O.run(pr.ora, prm, c["pos"], u, 5, S, G)  # a state with spread vision ranges
+    ref_state = st.get()
+    run, _ = make_run(pr, c, steps, stream=u)
+    run.steps(5)  # advance the device stream index to k = 5
+    run.write(pos=ref_state["pos"], luciferin=ref_state["luciferin"], score=ref_state["score"],
+              vision=ref_state["vision"], moved=ref_state["moved"], ref=ref_state["ref"])
+    # oracle: select + move of step 5 from the same state
+    O.lib().or_phase_select(prm, st.st, O._p(np.ascontiguousarray(u)), steps, 5, 0)
+    O.lib().or_phase_move(prm, st.st, 10, 10, 0)
+    want = st.get()
+    run.phases(2 | 4)
+    got = run.read()
+    assert np.array_equal(got["nnei"], want["nnei"].ravel())
+    assert np.array_equal(got["select"], want["select"].ravel())
+    assert np.array_equal(got["moved"], want["moved"].ravel())
+    assert np.array_equal(got["vision"], want["vision"].ravel())
+    assert want["moved"].sum() > 20
+    gp = got["pos"].reshape(want["pos"].shape)
+    # translation and extents are exact; the quaternion goes through acos/sin
+    assert np.array_equal(gp[..., :3], want["pos"][..., :3])
+    assert np.array_equal(gp[..., 7:], want["pos"][..., 7:])
+    assert np.allclose(gp[..., 3:7], want["pos"][..., 3:7], rtol=0, atol=POS_TOL)
+
+
+def test_fp64_run_tracks_oracle():
+    c = GC.case(10, 300, 160, 10, 10, 3, 48, swarm_radius=2.5)
+    pr = GC.Pair(c)
+    S, G, steps = 3, 48, 10
+    run, u = make_run(pr, c, steps)
+    st = O.State(S, G, pr.P, O.default_params(), c["pos"])
+    moved = 0
+    for k in range(steps):
+        run.steps(1)
+        O.lib().or_gso_step(pr.ora.c, O.default_params(), st.st, O._p(np.ascontiguousarray(u)),
+                            steps, k, 1, 0)
+        g, w = run.read(), st.get()
+        assert np.array_equal(g["select"], w["select"].ravel()), k
+        assert np.array_equal(g["nnei"], w["nnei"].ravel()), k
+        assert np.allclose(g["pos"].reshape(w["pos"].shape), w["pos"], rtol=0, atol=1e-11), k
+        assert np.allclose(g["score"], w["score"].ravel(), rtol=1e-12, atol=1e-12), k
+        assert np.allclose(g["luciferin"], w["luciferin"].ravel(), rtol=1e-12, atol=1e-12), k
+        moved += int(g["moved"].sum())
+    assert moved > 50
+
+
+def test_graph_replay_equals_eager():
+    c = GC.case(11, 400, 200, 10, 10, 3, 32, swarm_radius=3.0)
+    pr = GC.Pair(c)
+    outs = []
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        for graph in (False, True):
+            run, _ = make_run(pr, c, 6, form=form)
+            run.steps(2, graph=graph)
+            run.sync()
+            run.steps(4, graph=graph)
+            outs.append(run.read())
+            run.close()
+    for a, b in ((outs[0], outs[1]), (outs[2], outs[3])):
+        for key in ("pos", "luciferin", "score", "select", "pairs", "vision"):
+            assert np.array_equal(a[key], b[key]), key
+
+
+def test_batch_equals_solo_runs():
+    """Two complexes of different sizes in one run give the results of two solo runs."""
+    ca = GC.case(12, 350, 180, 10, 10, 2, 32, swarm_radius=3.0)
+    cb = GC.case(13, 520, 140, 10, 10, 3, 32, swarm_radius=3.0)
+    pa, pb = GC.Pair(ca), GC.Pair(cb)
+    steps = 4
+    ua, ub = O.stream(0, 2, steps, 32), O.stream(0, 3, steps, 32)
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        ra = sdgso.Run(pa.gpu, 2, 32, ca["pos"], ua, steps, form=form)
+        rb = sdgso.Run(pb.gpu, 3, 32, cb["pos"], ub, steps, form=form)
+        both = sdgso.Run([pa.gpu, pb.gpu], [2, 3], 32,
+                         np.concatenate([ca["pos"].ravel(), cb["pos"].ravel()]),
+                         np.concatenate([ua.ravel(), ub.ravel()]), steps, form=form)
+        for r in (ra, rb, both):
+            r.steps(steps)
+        a, b, ab = ra.read(), rb.read(), both.read()
+        for key in ("pos", "score", "luciferin", "select", "pairs"):
+            assert np.array_equal(np.concatenate([a[key], b[key]]), ab[key]), (form, key)
+
+
+def test_fp32_run_close_to_fp64():
+    c = GC.case(14, 600, 250, 10, 10, 4, 40, swarm_radius=4.0)
+    pr = GC.Pair(c)
+    r64, _ = make_run(pr, c, 5, form=sdgso.FP64)
+    r32, _ = make_run(pr, c, 5, form=sdgso.FP32_GUARD)
+    r64.steps(1)
+    r32.steps(1)
+    a, b = r64.read(), r32.read()
+    assert np.array_equal(a["pairs"], b["pairs"])
+    assert np.allclose(a["score"], b["score"], rtol=0, atol=1e-4)
+
+
+def test_step_past_stream_is_an_error():
+    c = GC.case(15, 100, 60, 0, 0, 1, 8)
+    pr = GC.Pair(c)
+    run, _ = make_run(pr, c, 2)
+    run.steps(2)
+    with pytest.raises(sdgso.SdgsoError) as e:
+        run.steps(1)
+    assert e.value.code == 4
