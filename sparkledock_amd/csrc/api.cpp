@@ -1,8 +1,8 @@
 // C-ABI entry points of libsdgso.so (declared in include/sdgso.h).
 //
 // Host responsibilities: argument validation, the per-complex preprocessing
-// (SoA layouts, mask folded into the modes, Morton-ordered receptor tiles and
-// ligand blocks for the FP32 guard form), device residency of every array, the
+// (SoA layouts, mask folded into the modes, receptor tiles and ligand blocks
+// grouped by recursive bisection, for the FP32 guard form), device residency of every array, the
 // step sequence on one HIP stream, optional hipGraph replay of a step, and
 // HIP-event timing of the phases.
 #include <hip/hip_runtime.h>
@@ -106,40 +106,43 @@ struct DevBuf {
     void upload(const std::vector<T> &v) { upload(v.data(), v.size()); }
 };
 
-// ---------------------------------------------------------------- Morton order
-static uint64_t spread3(uint64_t v) {
-    v &= 0x1fffff;
-    v = (v | v << 32) & 0x1f00000000ffffULL;
-    v = (v | v << 16) & 0x1f0000ff0000ffULL;
-    v = (v | v << 8) & 0x100f00f00f00f00fULL;
-    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
-    v = (v | v << 2) & 0x1249249249249249ULL;
-    return v;
+// Recursive coordinate bisection into groups of `size` atoms: split the set
+// along its longest extent so that the lower part holds a whole number of
+// groups; every group is full except the last.  Gives compact groups (small
+// bounding spheres) where a space-filling-curve order of a sparse chain does
+// not.
+static void rcb(const std::vector<double> &xyz, std::vector<int> &idx, size_t lo, size_t hi,
+                int size, std::vector<int> &out) {
+    const size_t n = hi - lo;
+    const size_t k = (n + size - 1) / size;
+    if (k <= 1) {
+        out.insert(out.end(), idx.begin() + lo, idx.begin() + hi);
+        return;
+    }
+    double mn[3] = {1e300, 1e300, 1e300}, mx[3] = {-1e300, -1e300, -1e300};
+    for (size_t i = lo; i < hi; i++)
+        for (int c = 0; c < 3; c++) {
+            mn[c] = std::min(mn[c], xyz[3 * (size_t)idx[i] + c]);
+            mx[c] = std::max(mx[c], xyz[3 * (size_t)idx[i] + c]);
+        }
+    int ax = 0;
+    for (int c = 1; c < 3; c++)
+        if (mx[c] - mn[c] > mx[ax] - mn[ax]) ax = c;
+    const size_t cut = lo + (k / 2) * size;
+    std::nth_element(idx.begin() + lo, idx.begin() + cut, idx.begin() + hi, [&](int a, int b) {
+        const double va = xyz[3 * (size_t)a + ax], vb = xyz[3 * (size_t)b + ax];
+        return va < vb || (va == vb && a < b);
+    });
+    rcb(xyz, idx, lo, cut, size, out);
+    rcb(xyz, idx, cut, hi, size, out);
 }
 
-// Atoms ordered along a Morton curve over their bounding box.
-static std::vector<int> morton_order(const std::vector<double> &xyz, int n) {
-    double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
-    for (int i = 0; i < n; i++)
-        for (int c = 0; c < 3; c++) {
-            lo[c] = std::min(lo[c], xyz[3 * i + c]);
-            hi[c] = std::max(hi[c], xyz[3 * i + c]);
-        }
-    double ext = 1e-9;
-    for (int c = 0; c < 3; c++) ext = std::max(ext, hi[c] - lo[c]);
-    std::vector<std::pair<uint64_t, int>> key(n);
-    for (int i = 0; i < n; i++) {
-        uint64_t code = 0;
-        for (int c = 0; c < 3; c++) {
-            const uint64_t q = (uint64_t)((xyz[3 * i + c] - lo[c]) / ext * 1048575.0);
-            code |= spread3(q) << c;
-        }
-        key[i] = {code, i};
-    }
-    std::sort(key.begin(), key.end());
-    std::vector<int> order(n);
-    for (int i = 0; i < n; i++) order[i] = key[i].second;
-    return order;
+static std::vector<int> group_order(const std::vector<double> &xyz, int n, int size) {
+    std::vector<int> idx(n), out;
+    for (int i = 0; i < n; i++) idx[i] = i;
+    out.reserve(n);
+    rcb(xyz, idx, 0, n, size, out);
+    return out;
 }
 
 }  // namespace sd
@@ -218,7 +221,7 @@ static void soa_inputs(int n, const double *xyz, const int *mask, int nm, const 
 static void build_tiles(sdgso_complex c, const double *xyz, const int *type, const std::vector<double> &msoa) {
     const int n = c->nr, nm = c->nmr;
     std::vector<double> v(xyz, xyz + 3 * (size_t)n);
-    std::vector<int> order = morton_order(v, n);
+    std::vector<int> order = group_order(v, n, kTile);
     const int ntiles = (n + kTile - 1) / kTile;
     const size_t slots = (size_t)ntiles * kTile;
     std::vector<float4> tiles(ntiles);
@@ -284,7 +287,7 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
 static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<double> &msoa) {
     const int n = c->nl, nm = c->nml;
     std::vector<double> v(xyz, xyz + 3 * (size_t)n);
-    std::vector<int> order = morton_order(v, n);
+    std::vector<int> order = group_order(v, n, kTile);
     const int nb = (n + kBlock - 1) / kBlock;
     std::vector<float4> blocks(nb);
     std::vector<int> perm((size_t)nb * kBlock, -1);

@@ -4,7 +4,7 @@
 // from a work list that keeps the blocks of one glowworm adjacent.  Lane l
 // owns ligand atom lperm[block*64 + l], transformed in FP64 in the reference
 // order (prepare_gso_tasks.cc:487-526) and kept in registers.  The receptor is
-// cut into Morton-ordered tiles of 64 atoms with bounding spheres; a tile is
+// cut into compact tiles of 64 atoms (recursive bisection) with bounding spheres; a tile is
 // visited only when its sphere, padded by the cutoff, the block sphere and the
 // normal-mode displacement bounds of this pose, reaches the block.  Inside a
 // visited tile each lane displaces one receptor atom (FP32 modes) and the

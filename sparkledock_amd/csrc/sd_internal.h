@@ -9,13 +9,13 @@
 //     rtype   i32 [nr]             type * 3360 (row offset into the table)
 //     ltype   i32 [nl]             type * 20
 //     table   f32 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
-//   receptor tiles (FP32 guard form): atoms in Morton order, 64 per tile
+//   receptor tiles (FP32 guard form): 64 atoms per tile, grouped by bisection
 //     tiles   f32x4 [ntiles]       tile centre (x,y,z) and radius
 //     rrel    f64 [3][ntiles*64]   coordinates - tile centre
 //     rmod32  f32 [nmr][3][ntiles*64]
 //     rtype_s i32 [ntiles*64]      type*3360 in tile order
 //     rperm   i32 [ntiles*64]      original atom index (-1 = padding)
-//   ligand blocks (FP32 guard form): atoms in Morton order, 64 per block
+//   ligand blocks (FP32 guard form): 64 atoms per block, grouped by bisection
 //     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
 //     lblocks f32x4 [nlblocks]     block centre in the ligand frame, radius
 //   run state, N = sum(swarms) * G glowworms
