@@ -9,6 +9,21 @@
 
 namespace sd {
 
+// Load through the global address space.  Pointers that come out of a
+// descriptor in memory are generic to the compiler, which then emits flat
+// loads (counted on both the vector-memory and LDS counters); every array the
+// kernels read lives in HBM.
+template <class T>
+__device__ __forceinline__ T gld(const T *p, size_t i) {
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+
+__device__ __forceinline__ float4 gld(const float4 *p, size_t i) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = ((const __attribute__((address_space(1))) f4 *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // inverse = conj(q) / |q|^2 (prepare_gso_tasks.cc:288-302)
 __device__ __forceinline__ void quat_inverse(const double q[4], double inv[4]) {
     const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];
@@ -40,15 +55,15 @@ __device__ __forceinline__ int bin_of_d(int d) { return max(0, min(d - 2, (d + 1
 // (prepare_gso_tasks.cc:465-482; masked atoms have zero mode rows).
 __device__ __forceinline__ void receptor_atom64(const DevComplex &c, const double *ext, int i,
                                                 double &x, double &y, double &z) {
-    x = c.r0[i];
-    y = c.r0[c.nr + i];
-    z = c.r0[2 * c.nr + i];
+    x = gld(c.r0, i);
+    y = gld(c.r0, c.nr + i);
+    z = gld(c.r0, 2 * c.nr + i);
     for (int m = 0; m < c.nmr; m++) {
         const double e = ext[m];
         const double *M = c.rmodes + (size_t)m * 3 * c.nr;
-        x += M[i] * e;
-        y += M[c.nr + i] * e;
-        z += M[2 * c.nr + i] * e;
+        x += gld(M, i) * e;
+        y += gld(M, c.nr + i) * e;
+        z += gld(M, 2 * c.nr + i) * e;
     }
 }
 
@@ -57,16 +72,16 @@ __device__ __forceinline__ void receptor_atom64(const DevComplex &c, const doubl
 __device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double *pose,
                                               const double inv[4], int j, double &x, double &y,
                                               double &z) {
-    x = c.l0[j];
-    y = c.l0[c.nl + j];
-    z = c.l0[2 * c.nl + j];
+    x = gld(c.l0, j);
+    y = gld(c.l0, c.nl + j);
+    z = gld(c.l0, 2 * c.nl + j);
     const double *ext = pose + 7 + c.nmr;
     for (int m = 0; m < c.nml; m++) {
         const double e = ext[m];
         const double *M = c.lmodes + (size_t)m * 3 * c.nl;
-        x += M[j] * e;
-        y += M[c.nl + j] * e;
-        z += M[2 * c.nl + j] * e;
+        x += gld(M, j) * e;
+        y += gld(M, c.nl + j) * e;
+        z += gld(M, 2 * c.nl + j) * e;
     }
     rotate_point(pose + 3, inv, x, y, z);
     x += pose[0];
@@ -95,7 +110,7 @@ __device__ __forceinline__ bool pair_term64(double rx, double ry, double rz, dou
     const double d2 = dx * dx + dy * dy + dz * dz;
     if (d2 <= kCutoff2 && d2 > 0.0) {
         const int d = (int)(sqrt(d2) * 2.0 - 1.0);
-        val = (double)table[ra + lb + bin_of_d(d)];
+        val = (double)gld(table, ra + lb + bin_of_d(d));
         return true;
     }
     return false;

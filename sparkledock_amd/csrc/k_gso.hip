@@ -190,6 +190,11 @@ hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s) {
     if (a.n_swarms <= 0) return hipSuccess;
     const int threads = ((a.G + 63) / 64) * 64;
     const size_t lds = sizeof(double) * ((size_t)4 * a.G + (size_t)threads * a.P);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)select_move_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(select_move_kernel, dim3(a.n_swarms), dim3(threads), lds, s, a);
     return hipGetLastError();
 }

@@ -42,7 +42,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         lx[j] = x;
         ly[j] = y;
         lz[j] = z;
-        lt[j] = c.ltype[j];
+        lt[j] = gld(c.ltype, j);
     }
     if (tid == 0) {
         double r[3];
@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         if (i < nr) {
             double rx, ry, rz;
             receptor_atom64(c, ext, i, rx, ry, rz);
-            const int ra = c.rtype[i];
+            const int ra = gld(c.rtype, i);
             for (int j = 0; j < nl; j++) {
                 double val;
                 if (pair_term64(rx, ry, rz, lx[j], ly[j], lz[j], ra, lt[j], table, val)) {
@@ -105,6 +105,11 @@ hipError_t launch_score64(const ScoreArgs &a, int max_nl, hipStream_t s) {
     if (a.n_glow <= 0) return hipSuccess;
     const size_t lds = sizeof(double) * (3 * (size_t)max_nl + 2 * kScore64Threads) +
                        sizeof(int) * (size_t)max_nl;
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute((const void *)score64_kernel,
+                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
     hipLaunchKernelGGL(score64_kernel, dim3(a.n_glow), dim3(kScore64Threads), lds, s, a);
     return hipGetLastError();
 }

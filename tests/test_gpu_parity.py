@@ -87,6 +87,50 @@ def test_bins_exact_on_lattice_boundaries():
     assert np.array_equal(p, p0)
 
 
+def boundary_sweep_case(seed=17):
+    """One receptor atom at the origin and ligand atoms at d2 = k^2/4 + delta
+    for every boundary k and offsets delta across the guard band, including
+    the window between the band edge and 225 + band rounded to FP32."""
+    rng = np.random.default_rng(seed)
+    deltas = np.concatenate([np.linspace(-2.5e-4, 2.5e-4, 21), np.linspace(0.85e-4, 1.25e-4, 17),
+                             -np.linspace(0.85e-4, 1.25e-4, 17), [0.0, 1e-9, -1e-9]])
+    d2 = np.array([k * k / 4.0 + d for k in range(2, 31) for d in deltas])
+    d2 = np.concatenate([d2, 225.0 + np.linspace(0.5e-4, 1.5e-4, 101)])
+    n = len(d2)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    c = dict(r_xyz=np.array([[0.0, 0.0, 0.0], [400.0, 0.0, 0.0]]),
+             l_xyz=dirs * np.sqrt(d2)[:, None],
+             r_type=np.array([5, 9], np.int32), l_type=rng.integers(0, 167, n).astype(np.int32),
+             table=(rng.normal(size=168 * 168 * 20) * 0.5).astype(np.float32),
+             r_modes=np.zeros((0, 2, 3)), l_modes=np.zeros((0, n, 3)),
+             r_mask=np.ones(2, np.int32), l_mask=np.ones(n, np.int32),
+             lig_ref=np.zeros(3))
+    c["pos"] = np.array([[[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0]]])
+    return c
+
+
+def test_fp32_guard_sweep_across_every_boundary():
+    c = boundary_sweep_case()
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, p0 = pr.oracle_scores(poses)
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        s, _, p = pr.gpu.score(poses, form)
+        assert np.array_equal(p, p0), form
+        if form == sdgso.FP64:
+            assert np.array_equal(s, s0)
+        else:
+            assert abs(s[0] - s0[0]) < 1e-5
+    want = []
+    for b in range(20):
+        t = np.zeros((168, 168, 20), np.float32)
+        t[:, :, b] = 1.0
+        want.append(round((4.7 - pr.with_table(t.reshape(-1)).score_pose(poses[0])[0]) / 0.0157))
+    got = per_bin_counts(pr, poses, sdgso.FP32_GUARD)[:, 0]
+    assert np.array_equal(got, np.array(want))
+
+
 def test_bins_exact_random_fp32():
     c = GC.case(8, 900, 400, 10, 10, 3, 8)
     pr = GC.Pair(c)
