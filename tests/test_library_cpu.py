@@ -97,3 +97,16 @@ def test_synthetic_table_and_poses():
     spread = np.linalg.norm(pos[:, :, :3] - pos[:, :1, :3] * 0 - centres[:, None, :] * 0, axis=2)
     assert spread.shape == (20, 50)
     assert np.abs(pos[:, :, 7:]).max() <= 1.0
+
+
+def test_device_bin_formula_matches_dist_to_bins():
+    """dev_common.h computes dist_to_bins[d] - 1 (interface.h:30-35) as
+    max(0, min(d - 2, (d + 11) >> 1)); check it for every reachable d (-1..29;
+    -1 arises for s < 1 where the kernels use (int)s - 1)."""
+    dist_to_bins = [1, 1, 1, 2, 3, 4, 5, 6, 7, 8, 9, 10, 11, 12, 13, 14, 14, 15, 15, 16, 16, 17,
+                    17, 18, 18, 19, 19, 20, 20, 21]
+    src = open(os.path.join(ROOT, "sparkledock_amd", "csrc", "dev_common.h")).read()
+    assert "return max(0, min(d - 2, (d + 11) >> 1));" in src
+    for d in range(-1, 30):
+        want = dist_to_bins[max(d, 0)] - 1
+        assert max(0, min(d - 2, (d + 11) >> 1)) == want, d
