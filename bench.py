@@ -142,6 +142,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--profile-step", action="store_true",
+                    help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
+                         "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the GSO path has no multi-GPU form in this build "
@@ -153,6 +156,19 @@ def main():
         sys.exit("no GPU visible")
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+
+    if args.profile_step:
+        x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.mul(x, 2.0)  # 1 GiB read + 1 GiB write: the counter calibration kernel
+        torch.cuda.synchronize()
+        del x, y
+        wl = workloads.build(args.workload, swarm_scale=args.swarm_scale, steps=2)
+        run = sdgso.Run(wl["complexes"], wl["n_swarms"], wl["G"], wl["pos"], wl["stream"],
+                        wl["steps"], form=sdgso.FP32_GUARD)
+        run.steps(1)
+        run.sync()
+        log("profile step done: %d glowworms" % (sum(wl["n_swarms"]) * wl["G"]))
+        return
 
     t0 = time.perf_counter()
     wl = workloads.build(args.workload, swarm_scale=args.swarm_scale,
@@ -167,6 +183,8 @@ def main():
     run.set_stream(stream.cuda_stream)
     run.steps(args.warmup)
     run.sync()
+    chosen, tune_ms = run.tune()
+    log("autotune: variant %d, ms per launch %s" % (chosen, tune_ms))
     run.profile(True)
     dt = timed_steps(run, args.steps, graph=False)
     prof = run.profile_read()
@@ -239,6 +257,7 @@ def main():
         "precision": prec,
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
         "pairs_per_eval": pairs_per_launch / n_glow,
+        "autotune": {"chosen": chosen, "ms_per_launch": tune_ms},
     }
     print(json.dumps(out), flush=True)
 

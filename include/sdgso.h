@@ -137,6 +137,18 @@ int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches);
  * stats[0] pairs recomputed in FP64. */
 int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
 
+/* Autotuner of the FP32 guard-band scoring kernel (no counterpart in the
+ * reference, whose tile sizes are compile-time constants,
+ * prepare_gso_task.cu:2503-2514).  sdgso_tune_variants() is the number of
+ * compiled variants (gathers in flight per lane x register/occupancy
+ * target); sdgso_run_tune scores the run's current poses with each variant
+ * `repeats` times (luciferin and step counters untouched), keeps the fastest
+ * for later steps, and returns the per-variant milliseconds per launch in
+ * times_ms[sdgso_tune_variants()] (may be NULL) and the choice in *chosen. */
+int sdgso_tune_variants(void);
+const char *sdgso_tune_variant_name(int variant);
+int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen);
+
 /* ---- host utilities (no device use) ---- */
 
 /* The reference's random stream: u[s][k][g] = successive

@@ -187,6 +187,7 @@ struct sdgso_run_s {
     hipStream_t stream_h = nullptr;
     hipGraphExec_t graph = nullptr;
     bool profiling = false;
+    int variant = 0;  // FP32 scoring kernel variant (sdgso_run_tune)
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_score, ev_move;
     double t_score = 0, t_move = 0;
     int64_t n_score = 0, n_move = 0;
@@ -468,7 +469,7 @@ static void enqueue_score(sdgso_run r, hipEvent_t ev0 = nullptr, hipEvent_t ev1 
         HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
         if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
     } else {
-        HIPCHK(launch_score32(a, r->stream_h));
+        HIPCHK(launch_score32(a, r->variant, r->stream_h));
         if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
         HIPCHK(launch_finish32(a, r->stream_h));
     }
@@ -584,6 +585,10 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
             for (int s = 0; s < n_swarms[k]; s++) scx.push_back(k);
             r->max_nl = std::max(r->max_nl, c->nl);
             r->max_lblocks = std::max(r->max_lblocks, c->dev.nlblocks);
+            if (form == SDGSO_FP32_GUARD && c->dev.ntiles > kMaxTiles32)
+                fail(SDGSO_EINVAL, "receptor too large for the FP32 form (" +
+                                       std::to_string(c->nr) + " atoms; at most " +
+                                       std::to_string(kMaxTiles32 * kTile) + ")");
             r->S += n_swarms[k];
         }
         r->P = 7 + r->nmr + r->nml;
@@ -811,6 +816,57 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats) {
     });
 }
 
+int sdgso_tune_variants(void) { return score32_variants(); }
+
+const char *sdgso_tune_variant_name(int v) { return score32_variant_name(v); }
+
+int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (repeats <= 0) repeats = 1;
+        ensure_device();
+        const int nv = score32_variants();
+        if (r->form != SDGSO_FP32_GUARD) {
+            if (times_ms)
+                for (int v = 0; v < nv; v++) times_ms[v] = 0;
+            if (chosen) *chosen = r->variant;
+            return;
+        }
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        // score the current poses with every variant; luciferin and step are
+        // left alone (the next step rescores these poses anyway)
+        ScoreArgs a = score_args(r);
+        a.update_luciferin = 0;
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        double best = 1e300;
+        int best_v = r->variant;
+        for (int v = 0; v < nv; v++) {
+            HIPCHK(launch_score32(a, v, r->stream_h));  // warm-up
+            HIPCHK(hipEventRecord(e0, r->stream_h));
+            for (int k = 0; k < repeats; k++) HIPCHK(launch_score32(a, v, r->stream_h));
+            HIPCHK(hipEventRecord(e1, r->stream_h));
+            HIPCHK(hipEventSynchronize(e1));
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+            const double t = ms / repeats;
+            if (times_ms) times_ms[v] = t;
+            if (t < best) {
+                best = t;
+                best_v = v;
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        HIPCHK(launch_finish32(a, r->stream_h));
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        drop_graph(r);
+        r->variant = best_v;
+        if (chosen) *chosen = best_v;
+    });
+}
+
 // ---------------------------------------------------------------- scoring plugin
 int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int form,
                       double guard_band, double *out_score, double *out_ref, int64_t *out_pairs) {
@@ -831,7 +887,7 @@ int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int for
         if (form == SDGSO_FP64) {
             HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
         } else {
-            HIPCHK(launch_score32(a, r->stream_h));
+            HIPCHK(launch_score32(a, r->variant, r->stream_h));
             HIPCHK(launch_finish32(a, r->stream_h));
         }
         rc = sdgso_run_read(r, nullptr, nullptr, out_score, nullptr, nullptr, nullptr, nullptr,

@@ -51,6 +51,39 @@ __device__ __forceinline__ void rotate_point(const double q[4], const double inv
 // 0,0,0,1,...,13 (d = 3..15), then pairs 13,14,14,...,19,19, then 20 at d = 29.
 __device__ __forceinline__ int bin_of_d(int d) { return max(0, min(d - 2, (d + 11) >> 1)); }
 
+// x/y/z += M[m][.][i] * e_m for m = 0..nm-1 in mode order (FP64, product
+// rounded, then added: prepare_gso_tasks.cc:465-504).  Modes are SoA
+// [nm][3][n]; the loads of four modes are issued before their adds so that
+// their latencies overlap, while the additions keep the reference order.
+__device__ __forceinline__ void modes_add64(const double *modes, int n, int i, const double *ext,
+                                            int nm, double &x, double &y, double &z) {
+    int m = 0;
+    for (; m + 4 <= nm; m += 4) {
+        double mx[4], my[4], mz[4], e[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double *M = modes + (size_t)(m + k) * 3 * n;
+            mx[k] = gld(M, i);
+            my[k] = gld(M, n + i);
+            mz[k] = gld(M, 2 * n + i);
+            e[k] = ext[m + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            x += mx[k] * e[k];
+            y += my[k] * e[k];
+            z += mz[k] * e[k];
+        }
+    }
+    for (; m < nm; m++) {
+        const double *M = modes + (size_t)m * 3 * n;
+        const double e = ext[m];
+        x += gld(M, i) * e;
+        y += gld(M, n + i) * e;
+        z += gld(M, 2 * n + i) * e;
+    }
+}
+
 // Receptor atom i displaced by the normal modes, FP64, reference order
 // (prepare_gso_tasks.cc:465-482; masked atoms have zero mode rows).
 __device__ __forceinline__ void receptor_atom64(const DevComplex &c, const double *ext, int i,
@@ -58,13 +91,7 @@ __device__ __forceinline__ void receptor_atom64(const DevComplex &c, const doubl
     x = gld(c.r0, i);
     y = gld(c.r0, c.nr + i);
     z = gld(c.r0, 2 * c.nr + i);
-    for (int m = 0; m < c.nmr; m++) {
-        const double e = ext[m];
-        const double *M = c.rmodes + (size_t)m * 3 * c.nr;
-        x += gld(M, i) * e;
-        y += gld(M, c.nr + i) * e;
-        z += gld(M, 2 * c.nr + i) * e;
-    }
+    modes_add64(c.rmodes, c.nr, i, ext, c.nmr, x, y, z);
 }
 
 // Ligand atom j of a pose: modes, rotation, translation
@@ -75,14 +102,7 @@ __device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double 
     x = gld(c.l0, j);
     y = gld(c.l0, c.nl + j);
     z = gld(c.l0, 2 * c.nl + j);
-    const double *ext = pose + 7 + c.nmr;
-    for (int m = 0; m < c.nml; m++) {
-        const double e = ext[m];
-        const double *M = c.lmodes + (size_t)m * 3 * c.nl;
-        x += gld(M, j) * e;
-        y += gld(M, c.nl + j) * e;
-        z += gld(M, 2 * c.nl + j) * e;
-    }
+    modes_add64(c.lmodes, c.nl, j, pose + 7 + c.nmr, c.nml, x, y, z);
     rotate_point(pose + 3, inv, x, y, z);
     x += pose[0];
     y += pose[1];

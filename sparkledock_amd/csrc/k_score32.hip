@@ -3,50 +3,60 @@
 // Work unit: one wavefront per (glowworm, ligand block of 64 atoms), taken
 // from a work list that keeps the blocks of one glowworm adjacent.  Lane l
 // owns ligand atom lperm[block*64 + l], transformed in FP64 in the reference
-// order (prepare_gso_tasks.cc:487-526) and kept in registers.  The receptor is
-// cut into compact tiles of 64 atoms (recursive bisection) with bounding
-// spheres; a tile is visited only when its sphere, padded by the cutoff, the
-// block sphere and the normal-mode displacement bounds of this pose, reaches
-// the block.  In a visited tile each lane displaces one receptor atom (FP32
-// modes) and the wave keeps only atoms that can reach the block (ballot).
+// order (prepare_gso_tasks.cc:487-526) and rounded once to a 2^-20 A
+// fixed-point grid whose origin is the block centre.
 //
-// Phase A (filter): the kept receptor atoms are broadcast one at a time with
-// readlane; every lane forms d2 with its ligand atom and the pairs inside the
-// cutoff (plus the guard band) are appended to a per-wave queue in LDS
-// (ballot + mbcnt compaction: d2, table row, pair tag).
-// Phase B (evaluate): when the queue fills, every lane takes queue entries
-// (all 64 lanes busy), finds the bin, issues the table gathers of several
-// entries back to back and only then adds them (compensated FP32 sums), so
-// the L2 latency of one gather overlaps the others.
+// Receptor tiles: compact groups of 64 atoms (recursive bisection) with
+// bounding spheres.  The wave tests 64 tile spheres per instruction against
+// the block sphere, padded by the cutoff and this pose's normal-mode
+// displacement bounds, and lists the candidates in LDS; it visits them in
+// order while the next tile's coordinates load.  In a visited tile each lane
+// displaces one receptor atom (FP32 modes, loads batched four modes deep),
+// rounds it to the same grid and the wave keeps the atoms that can reach the
+// block (ballot).
 //
-// Distances: both atoms are placed on a 2^-20 A fixed-point grid relative to
-// the tile centre (FP64 coordinates rounded once), so coordinate differences
-// are exact integers and convert to FP32 exactly below 16 A.  d2 in FP32 then
-// differs from the FP64 reference d2 by at most 9.5e-5 A^2, under the 1e-4
-// A^2 default band.  A pair whose FP32 d2 lies within the band of a boundary
-// k^2/4 (k = 0..30: d2 = 0, every bin edge and the 225 A^2 cutoff) is
-// recomputed in FP64 from the reference-order coordinates, so every bin and
-// the in/out decision equal the reference's.  A pair outside every band is
-// inside iff d2 < 225, and its bin is floor(2 sqrt(d2)) mapped through
-// dist_to_bins (raw v_sqrt_f32, 1 ulp, cannot cross an integer there).
+// Filter (two variants, chosen by the run's autotuner):
+//   readlane: the kept receptor atoms are broadcast one at a time; every lane
+//     forms the exact fixed-point d2 with its ligand atom and in-cutoff pairs
+//     are appended to a per-wave LDS queue (d2, table row, tag);
+//   mfma: the kept atoms are packed into LDS and d2 = |r|^2 + |l|^2 - 2 r.l of
+//     32 x 32 pairs is three v_mfma_f32_32x32x2f32 (r' = (x, y, z, |r|^2, 1, 0),
+//     l' = (-2x, -2y, -2z, 1, |l|^2, 0)); its FP32 cancellation error is below
+//     2e-3 A^2 in the block frame, so pairs with d2 <= 225 + 2 band + 0.05 are
+//     queued (receptor, ligand) and evaluated exactly.
 //
-// Energies: per-lane compensated (Kahan) FP32 sums; lanes are combined in
-// FP64 and written as one partial per (glowworm, block); finish32 adds a
-// glowworm's partials in block order in FP64.
+// Evaluate (all 64 lanes busy): exact d2 from the fixed-point coordinates
+// (integer differences are exact and convert to FP32 exactly below 16 A), the
+// bin, and the table value; several gathers are in flight per lane before the
+// compensated (Kahan) FP32 adds.  The FP32 d2 differs from the FP64 reference
+// d2 by at most 9.5e-5 A^2, under the default 1e-4 A^2 band; a pair within the
+// band of a boundary k^2/4 (k = 0..30: d2 = 0, every bin edge and the 225 A^2
+// cutoff) is recomputed in FP64 from the reference-order coordinates, so every
+// bin and in/out decision equals the reference's.  Outside every band a pair
+// is inside iff d2 < 225 and its bin is floor(2 sqrt(d2)) through dist_to_bins
+// (raw v_sqrt_f32, 1 ulp, cannot cross an integer there).
+//
+// Lanes are combined in FP64 and written as one partial per (glowworm,
+// block); finish32 adds a glowworm's partials in block order in FP64.
 #include "dev_common.h"
 
 namespace sd {
 
+typedef float floatx16 __attribute__((ext_vector_type(16)));
+
 constexpr int kScore32Threads = 256;
 constexpr int kWaves = kScore32Threads / 64;
-constexpr int kQueue = 256;                   // queued pairs per wave (phase A -> B)
-constexpr int kUnroll = kQueue / 64;          // gathers in flight per lane in phase B
+constexpr int kQueueR = 256;                  // readlane variant: queued pairs per wave
+constexpr int kQueueM = 1536;                 // mfma variant: room for one 32x32 group
+constexpr int kMaxTiles = kMaxTiles32;        // receptor tiles per complex (host-checked)
 constexpr float kQ = 1048576.0f;              // 2^20 units per A
+constexpr float kInvQ = 1.0f / 1048576.0f;
 constexpr float kInvQ2x2 = 1.0f / 524288.0f;  // sqrt(d2q) * 2^-19 = 2*sqrt(d2)
 constexpr float kQ2over4 = 274877906944.0f;   // 2^38 = 2^40 / 4
 constexpr float kQ2 = 1099511627776.0f;       // 2^40
 constexpr float k225q = 225.0f * kQ2;         // the cutoff in 2^-40 A^2 units (exact)
-constexpr int kFar = 1 << 29;
+constexpr float kFilterMargin = 0.05f;        // A^2, over the MFMA filter's rounding
+constexpr int kFar = 1 << 29;                 // far-away fixed-point coordinate (512 A)
 constexpr int kZeroIdx = SDGSO_TABLE_LEN + SDGSO_NBINS - 1;  // a zero of the table pad
 
 struct Kahan {
@@ -60,12 +70,11 @@ struct Kahan {
     }
 };
 
-// FP64 decision for a queued pair inside the guard band of a boundary.
-// The pair is rebuilt from scratch (pose transform of both atoms), so no
-// per-lane state crosses the call.
-// Returns (table value, 1) for a pair inside (0, 225], else (0, 0).
+// FP64 decision for a pair inside the guard band of a boundary, rebuilt from
+// the pose: returns (table value, 1) for a pair inside (0, 225], else (0, 0).
 __device__ __noinline__ float2 guard_pair64(const DevComplex &c, const double *pose,
                                             int lig_atom, int rec_atom) {
+    if (lig_atom < 0 || rec_atom < 0) return make_float2(0.f, 0.f);
     double inv[4];
     quat_inverse(pose + 3, inv);
     double Rx, Ry, Rz, Lx, Ly, Lz, v64 = 0.0;
@@ -76,167 +85,163 @@ __device__ __noinline__ float2 guard_pair64(const DevComplex &c, const double *p
     return make_float2(hit ? (float)v64 : 0.f, hit ? 1.f : 0.f);
 }
 
-__global__ __launch_bounds__(kScore32Threads) void score32_kernel(ScoreArgs a) {
-    __shared__ float q_d2[kWaves][kQueue];
-    __shared__ int q_row[kWaves][kQueue];
-    __shared__ int q_tag[kWaves][kQueue];
-    const int lane = threadIdx.x & 63;
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wv = blockIdx.x * kWaves + w;
-    if (wv >= a.n_work) return;
-    const int gid = a.work[2 * wv];
-    const int b = a.work[2 * wv + 1];
-    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
-    const double *pose = a.pos + (size_t)gid * a.P;
-    const double *rext = pose + 7;
-    const double *lext = pose + 7 + c.nmr;
-    double inv[4];
-    quat_inverse(pose + 3, inv);
+__device__ __forceinline__ int qround(double v) { return (int)rint(v * (double)kQ); }
 
-    // this lane's ligand atom, FP64 in reference order
-    const int j = gld(c.lperm, b * kBlock + lane);
-    double Lx = 0, Ly = 0, Lz = 0;
-    int lb = 0;
-    if (j >= 0) {
-        ligand_atom64(c, pose, inv, j, Lx, Ly, Lz);
-        lb = gld(c.ltype, j);
-    }
-    // block sphere in the docked frame and displacement bounds of this pose
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Classification of one exact fixed-point d2: bin of a pair outside every
+// guard band (take), or a pair inside a band (near: decided in FP64).
+struct PairClass {
+    bool take, near;
+    int bin;
+};
+
+__device__ __forceinline__ PairClass classify(float d2q, float bandq) {
+    const float s = __builtin_amdgcn_sqrtf(d2q) * kInvQ2x2;
+    const float kk = __builtin_rintf(s);
+    const float diff = __builtin_fmaf(kk * kk, -kQ2over4, d2q);
+    PairClass p;
+    // boundaries are k = 0..30; a pair beyond 225 + band is simply out
+    p.near = s < 30.5f && fabsf(diff) < bandq;
+    p.take = !p.near && d2q <= k225q;
+    p.bin = bin_of_d((int)s - 1);
+    return p;
+}
+
+__device__ __forceinline__ float d2_fixed(int4 a, int4 b) {
+    const float fx = (float)(a.x - b.x);
+    const float fy = (float)(a.y - b.y);
+    const float fz = (float)(a.z - b.z);
+    return __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
+}
+
+// Per-wave state shared by both filter variants.
+struct WaveCtx {
+    int lane, gid, b, nblk, ncand;
+    const double *pose;
+    double cb[3];     // block centre (fixed-point origin)
+    float fc[3];
+    float reach;      // cutoff + block radius + displacement bounds
+    int4 L;           // this lane's ligand atom: fixed point + type*20
+    int lig;          // its original index (-1 = padding)
+};
+
+// Set-up common to both variants: pose, block sphere, ligand atom, receptor
+// extents (LDS), candidate tiles (LDS).  Returns false for an idle wave.
+__device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex &c, int gid,
+                                           int b, float *ef, unsigned short *ct, WaveCtx &x) {
+    x.lane = threadIdx.x & 63;
+    x.gid = gid;
+    x.b = b;
+    x.pose = a.pos + (size_t)gid * a.P;
+    const double *rext = x.pose + 7;
+    const double *lext = x.pose + 7 + c.nmr;
+    double inv[4];
+    quat_inverse(x.pose + 3, inv);
     const float4 blk = gld(c.lblocks, b);
-    double cbx = blk.x, cby = blk.y, cbz = blk.z;
-    rotate_point(pose + 3, inv, cbx, cby, cbz);
-    cbx += pose[0];
-    cby += pose[1];
-    cbz += pose[2];
+    x.cb[0] = blk.x;
+    x.cb[1] = blk.y;
+    x.cb[2] = blk.z;
+    rotate_point(x.pose + 3, inv, x.cb[0], x.cb[1], x.cb[2]);
+    x.cb[0] += x.pose[0];
+    x.cb[1] += x.pose[1];
+    x.cb[2] += x.pose[2];
+    x.lig = gld(c.lperm, b * kBlock + x.lane);
+    // padding lanes sit at -512 A on every axis, padding receptor rows at +512 A
+    x.L = make_int4(-kFar, -kFar, -kFar, 0);
+    if (x.lig >= 0) {
+        double px, py, pz;
+        ligand_atom64(c, x.pose, inv, x.lig, px, py, pz);
+        x.L = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
+                        gld(c.ltype, x.lig));
+    }
+    x.nblk = __popcll(__ballot(x.lig >= 0));
+    if (x.lane < c.nmr) ef[x.lane] = (float)rext[x.lane];
     float dr = 0.f, dl = 0.f;
     for (int m = 0; m < c.nmr; m++) dr += fabsf((float)rext[m]) * c.rmode_max[m];
     for (int m = 0; m < c.nml; m++) dl += fabsf((float)lext[m]) * c.lmode_max[m];
     dr = dr * 1.0001f + 1e-3f;
     dl = dl * 1.0001f + 1e-3f;
-    const float reach_atom = 15.0f + blk.w + dl + dr + 0.01f;
-    const float fcx = (float)cbx, fcy = (float)cby, fcz = (float)cbz;
-
-    const float band = a.band;
-    const float cutq = (225.0f + 2.0f * band) * kQ2;
-    const float bandq = band * kQ2;
-    const int slots = c.ntiles * kTile;
-    float *qd = q_d2[w];
-    int *qr = q_row[w];
-    int *qt = q_tag[w];
-
-    Kahan acc;
-    int npairs = 0;
-    unsigned int nguard = 0;
-    int qn = 0;  // wave-uniform queue length
-
-    // Phase B: evaluate the queued pairs with all lanes.
-    auto flush = [&]() {
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        float v[kUnroll];
-        bool take[kUnroll], near[kUnroll];
-        int tag[kUnroll];
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            const int e = u * 64 + lane;
-            const bool valid = e < qn;
-            const float d2q = valid ? qd[e] : 0.f;
-            const int row = valid ? qr[e] : 0;
-            tag[u] = valid ? qt[e] : 0;
-            const float s = __builtin_amdgcn_sqrtf(d2q) * kInvQ2x2;
-            const float kk = __builtin_rintf(s);
-            const float diff = __builtin_fmaf(kk * kk, -kQ2over4, d2q);
-            near[u] = valid && fabsf(diff) < bandq;
-            take[u] = valid && !near[u] && d2q <= k225q;
-            const int idx = take[u] ? row + bin_of_d((int)s - 1) : kZeroIdx;
-            v[u] = gld(c.table, idx);
+    x.reach = 15.0f + blk.w + dl + dr + 0.01f;
+    x.fc[0] = (float)x.cb[0];
+    x.fc[1] = (float)x.cb[1];
+    x.fc[2] = (float)x.cb[2];
+    // candidate tiles: 64 sphere tests per instruction, listed in LDS
+    int n = 0;
+    for (int t0 = 0; t0 < c.ntiles; t0 += 64) {
+        const int t = t0 + x.lane;
+        bool ok = false;
+        if (t < c.ntiles) {
+            const float4 tl = gld(c.tiles, t);
+            const float tx = tl.x - x.fc[0], ty = tl.y - x.fc[1], tz = tl.z - x.fc[2];
+            const float lim = x.reach + tl.w;
+            ok = tx * tx + ty * ty + tz * tz <= lim * lim;
         }
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            acc.add(v[u], take[u]);
-            npairs += take[u] ? 1 : 0;
-            if (near[u]) {
-                const int t = tag[u];
-                const float2 r = guard_pair64(c, pose, gld(c.lperm, b * kBlock + (t >> 24)),
-                                              gld(c.rperm, t & 0xffffff));
-                const bool hit = r.y != 0.f;
-                acc.add(r.x, hit);
-                npairs += hit ? 1 : 0;
-                nguard++;
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-    };
-
-    for (int t = 0; t < c.ntiles; t++) {
-        const float4 tl = gld(c.tiles, t);
-        const float tx = tl.x - fcx, ty = tl.y - fcy, tz = tl.z - fcz;
-        const float lim = reach_atom + tl.w;
-        if (tx * tx + ty * ty + tz * tz > lim * lim) continue;
-        const int slot = t * kTile + lane;
-        const double rx = gld(c.rrel, slot), ry = gld(c.rrel, slots + slot),
-                     rz = gld(c.rrel, 2 * slots + slot);
-        const float ax = (float)rx + tx, ay = (float)ry + ty, az = (float)rz + tz;
-        const bool nearb = ax * ax + ay * ay + az * az <= reach_atom * reach_atom;
-        unsigned long long mask = __ballot(nearb);
-        if (!mask) continue;
-        int Rqx = kFar, Rqy = kFar, Rqz = kFar, ra = 0;
-        if (nearb) {
-            float Dx = 0.f, Dy = 0.f, Dz = 0.f;
-            for (int m = 0; m < c.nmr; m++) {
-                const float e = (float)rext[m];
-                const float *M = c.rmod32 + (size_t)m * 3 * slots;
-                Dx = __builtin_fmaf(gld(M, slot), e, Dx);
-                Dy = __builtin_fmaf(gld(M, slots + slot), e, Dy);
-                Dz = __builtin_fmaf(gld(M, 2 * slots + slot), e, Dz);
-            }
-            Rqx = (int)rint((rx + (double)Dx) * (double)kQ);
-            Rqy = (int)rint((ry + (double)Dy) * (double)kQ);
-            Rqz = (int)rint((rz + (double)Dz) * (double)kQ);
-            ra = gld(c.rtype_s, slot);
-        }
-        int Lqx = kFar, Lqy = kFar, Lqz = kFar;
-        if (j >= 0) {
-            Lqx = (int)rint((Lx - (double)tl.x) * (double)kQ);
-            Lqy = (int)rint((Ly - (double)tl.y) * (double)kQ);
-            Lqz = (int)rint((Lz - (double)tl.z) * (double)kQ);
-        }
-        const int row_l = lb;
-        const int tag_l = (lane << 24) | (t * kTile);
-        // Phase A: filter the kept receptor atoms against all 64 ligand lanes
-        while (mask) {
-            const int i = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            const int sx = __builtin_amdgcn_readlane(Rqx, i);
-            const int sy = __builtin_amdgcn_readlane(Rqy, i);
-            const int sz = __builtin_amdgcn_readlane(Rqz, i);
-            const int sa = __builtin_amdgcn_readlane(ra, i);
-            const float fx = (float)(Lqx - sx);
-            const float fy = (float)(Lqy - sy);
-            const float fz = (float)(Lqz - sz);
-            const float d2q = __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx));
-            const bool in = d2q <= cutq;
-            const unsigned long long m = __ballot(in);
-            if (m) {
-                if (in) {
-                    const int pos = qn + __builtin_amdgcn_mbcnt_hi(
-                                             (unsigned)(m >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    qd[pos] = d2q;
-                    qr[pos] = sa + row_l;
-                    qt[pos] = tag_l + i;
-                }
-                qn += __popcll(m);
-                if (qn > kQueue - 64) {
-                    flush();
-                    qn = 0;
-                }
-            }
-        }
+        const unsigned long long m = __ballot(ok);
+        if (ok)
+            ct[n + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                (unsigned short)t;
+        n += __popcll(m);
     }
-    if (qn) flush();
+    x.ncand = n;
+    wave_sync_lds();
+    return true;
+}
 
+// Tile visit: this lane's receptor atom of tile t (rel. coordinates already
+// loaded): near flag and, for near lanes, its displaced fixed-point position
+// in the block frame and table row.
+__device__ __forceinline__ bool tile_atom(const DevComplex &c, const WaveCtx &x, const float *ef,
+                                          int t, float4 tl, double rx, double ry, double rz,
+                                          int4 &R) {
+    const int slots = c.ntiles * kTile;
+    const int slot = t * kTile + x.lane;
+    const float ax = (float)rx + (tl.x - x.fc[0]);
+    const float ay = (float)ry + (tl.y - x.fc[1]);
+    const float az = (float)rz + (tl.z - x.fc[2]);
+    const bool nearb = ax * ax + ay * ay + az * az <= x.reach * x.reach;
+    R = make_int4(kFar, kFar, kFar, 0);
+    if (nearb) {
+        float Dx = 0.f, Dy = 0.f, Dz = 0.f;
+        int m = 0;
+        for (; m + 4 <= c.nmr; m += 4) {
+            float mx[4], my[4], mz[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float *M = c.rmod32 + (size_t)(m + k) * 3 * slots;
+                mx[k] = gld(M, slot);
+                my[k] = gld(M, slots + slot);
+                mz[k] = gld(M, 2 * slots + slot);
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                Dx = __builtin_fmaf(mx[k], ef[m + k], Dx);
+                Dy = __builtin_fmaf(my[k], ef[m + k], Dy);
+                Dz = __builtin_fmaf(mz[k], ef[m + k], Dz);
+            }
+        }
+        for (; m < c.nmr; m++) {
+            const float *M = c.rmod32 + (size_t)m * 3 * slots;
+            Dx = __builtin_fmaf(gld(M, slot), ef[m], Dx);
+            Dy = __builtin_fmaf(gld(M, slots + slot), ef[m], Dy);
+            Dz = __builtin_fmaf(gld(M, 2 * slots + slot), ef[m], Dz);
+        }
+        R = make_int4(qround(((double)tl.x - x.cb[0]) + rx + (double)Dx),
+                      qround(((double)tl.y - x.cb[1]) + ry + (double)Dy),
+                      qround(((double)tl.z - x.cb[2]) + rz + (double)Dz),
+                      gld(c.rtype_s, slot));
+    }
+    return nearb;
+}
+
+__device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
+                                               const Kahan &acc, int npairs,
+                                               unsigned int nguard) {
     double e = (double)acc.sum - (double)acc.comp;
     long long np = npairs;
     for (int off = 32; off > 0; off >>= 1) {
@@ -245,11 +250,296 @@ __global__ __launch_bounds__(kScore32Threads) void score32_kernel(ScoreArgs a) {
     }
     unsigned int ng = nguard;
     for (int off = 32; off > 0; off >>= 1) ng += __shfl_xor(ng, off, 64);
-    if (lane == 0) {
-        a.partial[(size_t)gid * a.max_lblocks + b] = e;
-        a.partial_pairs[(size_t)gid * a.max_lblocks + b] = np;
+    if (x.lane == 0) {
+        a.partial[(size_t)x.gid * a.max_lblocks + x.b] = e;
+        a.partial_pairs[(size_t)x.gid * a.max_lblocks + x.b] = np;
         if (a.guard_count && ng) atomicAdd(a.guard_count, (unsigned long long)ng);
     }
+}
+
+// ---------------------------------------------------------------- readlane variant
+template <int kUnroll, int kMinWaves>
+__global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(ScoreArgs a) {
+    __shared__ float s_d2[kWaves][kQueueR];   // queued pairs: fixed-point d2
+    __shared__ int s_row[kWaves][kQueueR];    //   table row (a*3360 + b*20)
+    __shared__ int s_tag[kWaves][kQueueR];    //   (ligand lane << 24) | receptor atom
+    __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
+    __shared__ unsigned short s_ct[kWaves][kMaxTiles];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = blockIdx.x * kWaves + w;
+    if (wv >= a.n_work) return;
+    const int gid = a.work[2 * wv];
+    const int b = a.work[2 * wv + 1];
+    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
+    float *ef = s_ef[w];
+    unsigned short *ct = s_ct[w];
+    float *qd = s_d2[w];
+    int *qr = s_row[w];
+    int *qt = s_tag[w];
+    WaveCtx x;
+    wave_setup(a, c, gid, b, ef, ct, x);
+    const int lane = x.lane;
+    const float bandq = a.band * kQ2;
+    const float cutq = (225.0f + 2.0f * a.band) * kQ2;
+    const int slots = c.ntiles * kTile;
+
+    Kahan acc;
+    int npairs = 0;
+    unsigned int nguard = 0;
+    int qn = 0;
+    auto flush = [&]() {
+        wave_sync_lds();
+        for (int base = 0; base < qn; base += 64 * kUnroll) {
+            float v[kUnroll];
+            bool take[kUnroll], near[kUnroll];
+            int tag[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int e = base + u * 64 + lane;
+                const bool valid = e < qn;
+                const int ec = min(e, kQueueR - 1);
+                const float d2q = qd[ec];
+                const int row = qr[ec];
+                tag[u] = qt[ec];
+                const PairClass pc = classify(d2q, bandq);
+                near[u] = valid && pc.near;
+                take[u] = valid && pc.take;
+                v[u] = gld(c.table, take[u] ? row + pc.bin : kZeroIdx);
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                acc.add(v[u], take[u]);
+                npairs += take[u] ? 1 : 0;
+                if (near[u]) {
+                    const float2 g2 = guard_pair64(c, x.pose,
+                                                   gld(c.lperm, x.b * kBlock + (tag[u] >> 24)),
+                                                   tag[u] & 0xffffff);
+                    acc.add(g2.x, g2.y != 0.f);
+                    npairs += g2.y != 0.f ? 1 : 0;
+                    nguard++;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        qn = 0;
+    };
+
+    int t_next = 0;
+    float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f);
+    double rx_next = 0, ry_next = 0, rz_next = 0;
+    if (x.ncand) {
+        t_next = __builtin_amdgcn_readfirstlane((int)ct[0]);
+        tl_next = gld(c.tiles, t_next);
+        rx_next = gld(c.rrel, t_next * kTile + lane);
+        ry_next = gld(c.rrel, slots + t_next * kTile + lane);
+        rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
+    }
+    for (int k = 0; k < x.ncand; k++) {
+        const int t = t_next;
+        const float4 tl = tl_next;
+        const double rx = rx_next, ry = ry_next, rz = rz_next;
+        if (k + 1 < x.ncand) {
+            t_next = __builtin_amdgcn_readfirstlane((int)ct[k + 1]);
+            tl_next = gld(c.tiles, t_next);
+            rx_next = gld(c.rrel, t_next * kTile + lane);
+            ry_next = gld(c.rrel, slots + t_next * kTile + lane);
+            rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
+        }
+        int4 R;
+        const bool nearb = tile_atom(c, x, ef, t, tl, rx, ry, rz, R);
+        unsigned long long mask = __ballot(nearb);
+        const int rorig = nearb ? gld(c.rperm, t * kTile + lane) : 0;
+        const int tag_l = lane << 24;
+        while (mask) {
+            const int i = __ffsll((long long)mask) - 1;
+            mask &= mask - 1;
+            const int4 Ri = make_int4(__builtin_amdgcn_readlane(R.x, i),
+                                      __builtin_amdgcn_readlane(R.y, i),
+                                      __builtin_amdgcn_readlane(R.z, i), 0);
+            const int sa = __builtin_amdgcn_readlane(R.w, i);
+            const float d2q = d2_fixed(x.L, Ri);
+            const bool in = d2q <= cutq;
+            const unsigned long long m = __ballot(in);
+            if (m) {
+                if (in) {
+                    const int pos = qn + __builtin_amdgcn_mbcnt_hi(
+                                             (unsigned)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                    qd[pos] = d2q;
+                    qr[pos] = sa + x.L.w;
+                    qt[pos] = tag_l | __builtin_amdgcn_readlane(rorig, i);
+                }
+                qn += __popcll(m);
+                if (qn > kQueueR - 64) flush();
+            }
+        }
+    }
+    if (qn) flush();
+    write_partials(a, x, acc, npairs, nguard);
+}
+
+// ---------------------------------------------------------------- mfma variant
+template <int kUnroll, int kMinWaves>
+__global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(ScoreArgs a) {
+    __shared__ int4 s_rt[kWaves][64];     // packed receptor atoms of the tile: fixed point + row
+    __shared__ int s_ro[kWaves][64];      // their original indices
+    __shared__ int4 s_lg[kWaves][64];     // the block's ligand atoms: fixed point + type*20
+    __shared__ int s_q[kWaves][kQueueM];  // queued pairs: (packed receptor << 8) | ligand lane
+    __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
+    __shared__ unsigned short s_ct[kWaves][kMaxTiles];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = blockIdx.x * kWaves + w;
+    if (wv >= a.n_work) return;
+    const int gid = a.work[2 * wv];
+    const int b = a.work[2 * wv + 1];
+    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
+    int4 *rt = s_rt[w];
+    int *ro = s_ro[w];
+    int4 *lg = s_lg[w];
+    int *qq = s_q[w];
+    float *ef = s_ef[w];
+    unsigned short *ct = s_ct[w];
+    WaveCtx x;
+    wave_setup(a, c, gid, b, ef, ct, x);
+    const int lane = x.lane;
+    lg[lane] = x.L;
+    const float bandq = a.band * kQ2;
+    const float thr = 225.0f + 2.0f * a.band + kFilterMargin;
+    const int slots = c.ntiles * kTile;
+    const bool lo_half = lane < 32;
+    wave_sync_lds();
+    // B operands of the two ligand sub-blocks: l' = (-2x, -2y, -2z, 1, |l|^2, 0)
+    float bop[2][3];
+#pragma unroll
+    for (int g = 0; g < 2; g++) {
+        const int4 q = lg[32 * g + (lane & 31)];
+        const float px = (float)q.x * kInvQ, py = (float)q.y * kInvQ, pz = (float)q.z * kInvQ;
+        const float l2 = px * px + py * py + pz * pz;
+        bop[g][0] = lo_half ? -2.f * px : -2.f * py;
+        bop[g][1] = lo_half ? -2.f * pz : 1.f;
+        bop[g][2] = lo_half ? l2 : 0.f;
+    }
+
+    Kahan acc;
+    int npairs = 0;
+    unsigned int nguard = 0;
+    int qn = 0;
+    auto flush = [&]() {
+        wave_sync_lds();
+        for (int base = 0; base < qn; base += 64 * kUnroll) {
+            float v[kUnroll];
+            bool take[kUnroll], near[kUnroll];
+            int tag[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int e = base + u * 64 + lane;
+                const bool valid = e < qn;
+                const int raw = qq[min(e, kQueueM - 1)];
+                tag[u] = valid ? raw : 0;
+                const int4 r = rt[tag[u] >> 8];
+                const int4 l = lg[tag[u] & 63];
+                const PairClass pc = classify(d2_fixed(l, r), bandq);
+                near[u] = valid && pc.near;
+                take[u] = valid && pc.take;
+                v[u] = gld(c.table, take[u] ? r.w + l.w + pc.bin : kZeroIdx);
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                acc.add(v[u], take[u]);
+                npairs += take[u] ? 1 : 0;
+                if (near[u]) {
+                    const float2 g2 = guard_pair64(c, x.pose,
+                                                   gld(c.lperm, x.b * kBlock + (tag[u] & 63)),
+                                                   ro[tag[u] >> 8]);
+                    acc.add(g2.x, g2.y != 0.f);
+                    npairs += g2.y != 0.f ? 1 : 0;
+                    nguard++;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        qn = 0;
+    };
+
+    int t_next = 0;
+    float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f);
+    double rx_next = 0, ry_next = 0, rz_next = 0;
+    if (x.ncand) {
+        t_next = __builtin_amdgcn_readfirstlane((int)ct[0]);
+        tl_next = gld(c.tiles, t_next);
+        rx_next = gld(c.rrel, t_next * kTile + lane);
+        ry_next = gld(c.rrel, slots + t_next * kTile + lane);
+        rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
+    }
+    for (int k = 0; k < x.ncand; k++) {
+        const int t = t_next;
+        const float4 tl = tl_next;
+        const double rx = rx_next, ry = ry_next, rz = rz_next;
+        if (k + 1 < x.ncand) {
+            t_next = __builtin_amdgcn_readfirstlane((int)ct[k + 1]);
+            tl_next = gld(c.tiles, t_next);
+            rx_next = gld(c.rrel, t_next * kTile + lane);
+            ry_next = gld(c.rrel, slots + t_next * kTile + lane);
+            rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
+        }
+        int4 R;
+        const bool nearb = tile_atom(c, x, ef, t, tl, rx, ry, rz, R);
+        const unsigned long long nmask = __ballot(nearb);
+        if (!nmask) continue;
+        const int n_near = __popcll(nmask);
+        if (nearb) {
+            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(nmask >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)nmask, 0u));
+            rt[pos] = R;
+            ro[pos] = gld(c.rperm, t * kTile + lane);
+        }
+        wave_sync_lds();
+        for (int h = 0; h * 32 < n_near; h++) {
+            // A operands of 32 packed receptor atoms: r' = (x, y, z, |r|^2, 1, 0)
+            const int rc = 32 * h + (lane & 31);
+            const int4 q = rc < n_near ? rt[rc] : make_int4(kFar, kFar, kFar, 0);
+            const float px = (float)q.x * kInvQ, py = (float)q.y * kInvQ, pz = (float)q.z * kInvQ;
+            const float r2 = px * px + py * py + pz * pz;
+            const float a0 = lo_half ? px : py;
+            const float a1 = lo_half ? pz : r2;
+            const float a2 = lo_half ? 1.f : 0.f;
+            // row of accumulator v in lane l: 8*(v/4) + 4*(l/32) + v%4 (column l%32)
+            const int rowbase = 32 * h + 4 * (lane >> 5);
+#pragma unroll 1
+            for (int g = 0; g < 2; g++) {
+                if (32 * g >= x.nblk) break;
+                const float b0 = g ? bop[1][0] : bop[0][0];
+                const float b1 = g ? bop[1][1] : bop[0][1];
+                const float b2 = g ? bop[1][2] : bop[0][2];
+                floatx16 d2 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                               0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                d2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, d2, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, d2, 0, 0, 0);
+                d2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, d2, 0, 0, 0);
+                const int col = 32 * g + (lane & 31);
+                // the queue has room for a whole 32 x 32 group (qn <= kQueueM - 1024)
+#pragma unroll
+                for (int v = 0; v < 16; v++) {
+                    const bool in = d2[v] <= thr;
+                    const unsigned long long m = __ballot(in);
+                    if (m) {
+                        if (in) {
+                            const int pos = qn + __builtin_amdgcn_mbcnt_hi(
+                                                     (unsigned)(m >> 32),
+                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                            qq[pos] = ((rowbase + 8 * (v >> 2) + (v & 3)) << 8) | col;
+                        }
+                        qn += __popcll(m);
+                    }
+                }
+                // flush when the next group might not fit, and before the packed
+                // tile in LDS is replaced by the next tile
+                const bool last = 32 * (h + 1) >= n_near && 32 * (g + 1) >= x.nblk;
+                if (qn > kQueueM - 1024 || (last && qn)) flush();
+            }
+        }
+    }
+    write_partials(a, x, acc, npairs, nguard);
 }
 
 // Per glowworm: energy = partials in block order (FP64), score, luciferin,
@@ -286,10 +576,34 @@ __global__ void advance_kernel(int *k) {
     if (threadIdx.x == 0) *k += 1;
 }
 
-hipError_t launch_score32(const ScoreArgs &a, hipStream_t s) {
+// Variant table of the FP32 scoring kernel: filter x (gathers in flight,
+// min waves/SIMD).
+namespace {
+struct Variant {
+    const char *name;
+    void (*kernel)(ScoreArgs);
+};
+const Variant kVariants[] = {
+    {"readlane/unroll4/occ4", score32r_kernel<4, 4>},
+    {"readlane/unroll2/occ5", score32r_kernel<2, 5>},
+    {"readlane/unroll4/occ3", score32r_kernel<4, 3>},
+    {"mfma/unroll2/occ4", score32m_kernel<2, 4>},
+    {"mfma/unroll4/occ3", score32m_kernel<4, 3>},
+};
+constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
+}  // namespace
+
+int score32_variants() { return kNumVariants; }
+
+const char *score32_variant_name(int v) {
+    return (v >= 0 && v < kNumVariants) ? kVariants[v].name : "?";
+}
+
+hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     if (a.n_work <= 0) return hipSuccess;
+    if (variant < 0 || variant >= kNumVariants) variant = 0;
     const int grid = (a.n_work + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(score32_kernel, dim3(grid), dim3(kScore32Threads), 0, s, a);
+    hipLaunchKernelGGL(kVariants[variant].kernel, dim3(grid), dim3(kScore32Threads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -38,6 +38,7 @@ constexpr int kTile = 64;              // receptor atoms per tile = wavefront wi
 constexpr int kBlock = 64;             // ligand atoms per block = wavefront width
 constexpr double kQuantScale = 1048576.0;  // 2^20: fixed-point units per A
 constexpr double kCutoff2 = 225.0;
+constexpr int kMaxTiles32 = 1024;      // FP32 form: receptor tiles per complex (65536 atoms)
 
 // Per-complex device descriptor (plain data; lives in a device array).
 struct DevComplex {
@@ -111,7 +112,9 @@ struct SelectMoveArgs {
 
 // Launchers (defined in the .hip files).
 hipError_t launch_score64(const ScoreArgs &a, int max_nl, hipStream_t s);
-hipError_t launch_score32(const ScoreArgs &a, hipStream_t s);
+hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s);
+int score32_variants();
+const char *score32_variant_name(int v);
 hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s);
 hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s);
 hipError_t launch_advance(int *d_k, hipStream_t s);

@@ -65,6 +65,9 @@ SIGNATURES = [
     ("sdgso_run_profile", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_run_profile_read", ctypes.c_int, [_VP, _D, _I64]),
     ("sdgso_run_guard_stats", ctypes.c_int, [_VP, _I64]),
+    ("sdgso_tune_variants", ctypes.c_int, []),
+    ("sdgso_tune_variant_name", ctypes.c_char_p, [ctypes.c_int]),
+    ("sdgso_run_tune", ctypes.c_int, [_VP, ctypes.c_int, _D, _I]),
     ("sdgso_stream", ctypes.c_int,
      [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     ("sdgso_synth_molecule", ctypes.c_int,
@@ -244,6 +247,16 @@ class Run:
         n = np.zeros(2, np.int64)
         _check(lib().sdgso_run_profile_read(self.h, _p(t), _p(n, _I64)))
         return dict(score_ms=t[0], move_ms=t[1], score_launches=int(n[0]), move_launches=int(n[1]))
+
+    def tune(self, repeats=1):
+        """Time every FP32 scoring-kernel variant on the current poses and keep
+        the fastest; returns (chosen, {variant name: ms per launch})."""
+        nv = lib().sdgso_tune_variants()
+        t = np.zeros(nv)
+        ch = ctypes.c_int(0)
+        _check(lib().sdgso_run_tune(self.h, int(repeats), _p(t), ctypes.byref(ch)))
+        names = [lib().sdgso_tune_variant_name(v).decode() for v in range(nv)]
+        return ch.value, dict(zip(names, t.tolist()))
 
     def guard_stats(self):
         s = np.zeros(1, np.int64)
