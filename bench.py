@@ -19,7 +19,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from sparkledock_amd import sdgso, workloads  # noqa: E402
+from sparkledock_amd import ranking, sdgso, workloads  # noqa: E402
 
 METRIC = "glowworm-pose evals/sec on 1 MI355X, plain FP64 vs batched FP32-guarded"
 FP32_PEAK_TFLOPS = 157.3
@@ -76,6 +76,32 @@ def precision_check(wl, n_per_complex=64, seed=5):
         start += S * G
     return dict(poses=n, pair_counts_equal=bins_equal, energy_rel_err_sum_abs=worst_rel,
                 score_rel_err=worst_rel_score)
+
+
+def ranking_overlap(stream, workload="c1", steps=30):
+    """End-to-end: the same seeded run in both forms for `steps` steps; overlap
+    of the top-100 glowworms ranked by final score and by luciferin
+    (analysis/lgd_rank.py:311-336 ordering)."""
+    wl = workloads.build(workload, steps=steps)
+    out = {}
+    finals = {}
+    for form in (sdgso.FP32_GUARD, sdgso.FP64):
+        r = sdgso.Run(wl["complexes"], wl["n_swarms"], wl["G"], wl["pos"], wl["stream"], steps,
+                      form=form)
+        r.set_stream(stream.cuda_stream)
+        if form == sdgso.FP32_GUARD:
+            r.tune()
+        r.steps(steps, graph=True)
+        finals[form] = r.read()
+        r.close()
+    a, b = finals[sdgso.FP32_GUARD], finals[sdgso.FP64]
+    out["workload"] = workload
+    out["steps"] = steps
+    out["top100_overlap_by_scoring"] = ranking.top_overlap(a["score"], b["score"], 100)
+    out["top100_overlap_by_luciferin"] = ranking.top_overlap(a["luciferin"], b["luciferin"], 100)
+    out["selections_equal_last_step"] = float(np.mean(a["select"] == b["select"]))
+    out["max_abs_pose_diff"] = float(np.max(np.abs(a["pos"] - b["pos"])))
+    return out
 
 
 def cpu_baseline(wl, threads, target_s=12.0):
@@ -142,6 +168,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--ranking-steps", type=int, default=30,
+                    help="steps of the end-to-end FP32-vs-FP64 ranking comparison on c1 (0 = skip)")
     ap.add_argument("--profile-step", action="store_true",
                     help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
                          "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
@@ -217,6 +245,10 @@ def main():
         log("fp64 plain: %.1f ms/step" % fp64["ms_per_step"])
 
     prec = precision_check(wl)
+    rank_cmp = None
+    if args.ranking_steps > 0:
+        rank_cmp = ranking_overlap(stream, "c1", args.ranking_steps)
+        log("ranking:", rank_cmp)
     log("precision:", prec)
 
     cpu = None
@@ -255,6 +287,7 @@ def main():
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
         "graph_ms_per_step": (1e3 * dtg) if dtg else None,
         "precision": prec,
+        "ranking": rank_cmp,
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
         "pairs_per_eval": pairs_per_launch / n_glow,
         "autotune": {"chosen": chosen, "ms_per_launch": tune_ms},
