@@ -23,6 +23,7 @@ from sparkledock_amd import ranking, sdgso, workloads  # noqa: E402
 
 METRIC = "glowworm-pose evals/sec on 1 MI355X, plain FP64 vs batched FP32-guarded"
 FP32_PEAK_TFLOPS = 157.3
+FP64_PEAK_TFLOPS = 78.6
 
 
 def log(*a):
@@ -170,6 +171,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ranking-steps", type=int, default=30,
                     help="steps of the end-to-end FP32-vs-FP64 ranking comparison on c1 (0 = skip)")
+    ap.add_argument("--variant", type=int, default=None,
+                    help="FP32 scoring-kernel variant to use instead of the autotuner's choice")
     ap.add_argument("--profile-step", action="store_true",
                     help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
                          "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
@@ -193,6 +196,8 @@ def main():
         wl = workloads.build(args.workload, swarm_scale=args.swarm_scale, steps=2)
         run = sdgso.Run(wl["complexes"], wl["n_swarms"], wl["G"], wl["pos"], wl["stream"],
                         wl["steps"], form=sdgso.FP32_GUARD)
+        if args.variant is not None:
+            run.set_variant(args.variant)
         run.steps(1)
         run.sync()
         log("profile step done: %d glowworms" % (sum(wl["n_swarms"]) * wl["G"]))
@@ -211,14 +216,19 @@ def main():
     run.set_stream(stream.cuda_stream)
     run.steps(args.warmup)
     run.sync()
-    chosen, tune_ms = run.tune()
-    log("autotune: variant %d, ms per launch %s" % (chosen, tune_ms))
+    if args.variant is None:
+        chosen, tune_ms = run.tune()
+        log("autotune: variant %d, ms per launch %s" % (chosen, tune_ms))
+    else:
+        run.set_variant(args.variant)
+        chosen, tune_ms = args.variant, None
     run.profile(True)
     dt = timed_steps(run, args.steps, graph=False)
     prof = run.profile_read()
     guard = run.guard_stats()
     run.profile(False)
     st = run.read()
+    filt = run.filter_stats()
     pairs_per_launch = float(st["pairs"].sum())
     score_ms = prof["score_ms"] / max(1, prof["score_launches"])
     log("fp32 guard: %.2f ms/step, score kernel %.2f ms, pairs/launch %.3e, guard recomputes %d"
@@ -239,9 +249,15 @@ def main():
             r64.set_stream(stream.cuda_stream)
             t64 += timed_steps(r64, args.fp64_steps, graph=False)
             r64.close()
+        # the brute-force kernel evaluates the difference-form d2 of every pair
+        all_pairs = sum(S * wl["G"] * cx.nr * cx.nl for cx, S in zip(wl["complexes"], wl["n_swarms"]))
+        tf64 = 8.0 * all_pairs * args.fp64_steps / t64 / 1e12
         fp64 = dict(value=n_glow * args.fp64_steps / t64,
                     ms_per_step=1e3 * t64 / args.fp64_steps, steps=args.fp64_steps,
-                    form="one run per complex, brute force over all pairs in FP64")
+                    form="one run per complex, brute force over all pairs in FP64",
+                    roofline={"bound": "fp64 valu", "achieved": tf64, "peak": FP64_PEAK_TFLOPS,
+                              "unit": "TFLOP/s", "frac": tf64 / FP64_PEAK_TFLOPS,
+                              "algorithmic": "8 FP64 flop per receptor-ligand pair, all Nr*Nl pairs"})
         log("fp64 plain: %.1f ms/step" % fp64["ms_per_step"])
 
     prec = precision_check(wl)
@@ -290,6 +306,7 @@ def main():
         "ranking": rank_cmp,
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
         "pairs_per_eval": pairs_per_launch / n_glow,
+        "filter": filt,
         "autotune": {"chosen": chosen, "ms_per_launch": tune_ms},
     }
     print(json.dumps(out), flush=True)

@@ -148,6 +148,17 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
 int sdgso_tune_variants(void);
 const char *sdgso_tune_variant_name(int variant);
 int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen);
+/* Use a given variant (e.g. one chosen by an earlier sdgso_run_tune of the
+ * same workload) without timing. */
+int sdgso_run_set_variant(sdgso_run r, int variant);
+
+/* Filter statistics of the FP32 guard-band scoring kernel (no counterpart in
+ * the reference, which scores every pair): scores the run's current poses
+ * once with the chosen variant, leaves the run state unchanged, and
+ * returns stats[6] = {waves (glowworm x ligand block), waves with a candidate
+ * receptor tile, candidate tiles visited, receptor atoms passing the atom
+ * test, pairs queued for exact evaluation, pairs inside the cutoff}. */
+int sdgso_run_filter_stats(sdgso_run r, int64_t *stats);
 
 /* ---- host utilities (no device use) ---- */
 

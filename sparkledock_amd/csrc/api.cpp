@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdint>
 #include <cstring>
 #include <memory>
 #include <random>
@@ -160,6 +161,10 @@ struct sdgso_complex_s {
     DevBuf<float4> tiles, lblocks;
     DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
+    DevBuf<float4> rpos32;     // [ntiles*64] f32 coordinates - tile centre
+    DevBuf<double> rrel4;      // [ntiles*64][4] f64 coordinates - tile centre
+    DevBuf<float> rmodA;       // [ntiles*64][rmod_stride] modes by slot
+    int rmod_stride = 0;
     DevBuf<int> rtype_s, rperm, lperm;
     DevComplex dev{};          // descriptor with device pointers
     double max_rec_extent = 0; // |xyz| maxima, for the guard-band bound
@@ -229,6 +234,11 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
     std::vector<double> rrel(3 * slots, 0.0);
     std::vector<float> m32((size_t)nm * 3 * slots, 0.0f);
     std::vector<int> ts(slots, 0), perm(slots, -1);
+    // whole groups of four modes (twelve floats) per slot
+    const int stride = (3 * nm + 11) / 12 * 12;
+    std::vector<float4> p32(slots);
+    std::vector<double> r4(4 * slots, 0.0);
+    std::vector<float> mA((size_t)stride * slots, 0.0f);
     for (int t = 0; t < ntiles; t++) {
         const int a0 = t * kTile, a1 = std::min(n, a0 + kTile);
         double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
@@ -257,12 +267,18 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
                 ts[s] = type[i] * SDGSO_NBINS * SDGSO_NTYPES;
                 for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = xyz[3 * (size_t)i + k] - (double)cf[k];
                 for (int m = 0; m < nm; m++)
-                    for (int k = 0; k < 3; k++)
-                        m32[((size_t)m * 3 + k) * slots + s] = (float)msoa[((size_t)m * 3 + k) * n + i];
+                    for (int k = 0; k < 3; k++) {
+                        const float mv = (float)msoa[((size_t)m * 3 + k) * n + i];
+                        m32[((size_t)m * 3 + k) * slots + s] = mv;
+                        mA[(size_t)s * stride + 3 * m + k] = mv;
+                    }
             } else {
                 // padding slot: far away, type 0, zero modes
                 for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = 1.0e4;
             }
+            for (int k = 0; k < 3; k++) r4[4 * (size_t)s + k] = rrel[(size_t)k * slots + s];
+            p32[s] = make_float4((float)r4[4 * (size_t)s], (float)r4[4 * (size_t)s + 1],
+                                 (float)r4[4 * (size_t)s + 2], 0.0f);
         }
     }
     for (int m = 0; m < nm; m++) {
@@ -280,6 +296,10 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
     c->tiles.upload(tiles);
     c->rrel.upload(rrel);
     c->rmod32.upload(m32);
+    c->rpos32.upload(p32);
+    c->rrel4.upload(r4);
+    c->rmodA.upload(mA);
+    c->rmod_stride = stride;
     c->rtype_s.upload(ts);
     c->rperm.upload(perm);
     c->dev.ntiles = ntiles;
@@ -406,6 +426,10 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         for (int k = 0; k < 3; k++) d.lref[k] = lig_ref[k];
         d.tiles = c->tiles.p;
         d.rmod32 = c->rmod32.p;
+        d.rpos32 = c->rpos32.p;
+        d.rrel4 = c->rrel4.p;
+        d.rmodA = c->rmodA.p;
+        d.rmod_stride = c->rmod_stride;
         d.rtype_s = c->rtype_s.p;
         d.rperm = c->rperm.p;
         d.lperm = c->lperm.p;
@@ -632,6 +656,13 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
             r->partial_pairs.alloc(N * (size_t)r->max_lblocks);
             // work list: (glowworm, ligand block) pairs, blocks of one glowworm adjacent
             std::vector<int> work;
+            size_t total = 0;
+            for (int k = 0; k < n_complex; k++)
+                total += (size_t)n_swarms[k] * r->G * r->cxs[k]->dev.nlblocks;
+            if (total > (size_t)INT32_MAX / 2)
+                fail(SDGSO_EINVAL, "run too large: " + std::to_string(total) +
+                                       " (glowworm, ligand block) pairs");
+            work.reserve(2 * total);
             size_t g0 = 0;
             for (int k = 0; k < n_complex; k++) {
                 const int nb = r->cxs[k]->dev.nlblocks;
@@ -864,6 +895,38 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
         drop_graph(r);
         r->variant = best_v;
         if (chosen) *chosen = best_v;
+    });
+}
+
+int sdgso_run_set_variant(sdgso_run r, int variant) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (variant < 0 || variant >= score32_variants())
+            fail(SDGSO_EINVAL, "variant out of [0, " + std::to_string(score32_variants()) + ")");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        if (variant != r->variant) drop_graph(r);
+        r->variant = variant;
+    });
+}
+
+int sdgso_run_filter_stats(sdgso_run r, int64_t *stats) {
+    return guarded([&] {
+        if (!r || !stats) fail(SDGSO_EINVAL, "null argument");
+        if (r->form != SDGSO_FP32_GUARD) fail(SDGSO_EINVAL, "filter statistics need the FP32 form");
+        ensure_device();
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        DevBuf<unsigned long long> st;
+        std::vector<unsigned long long> h(6, 0);
+        st.upload(h);
+        ScoreArgs a = score_args(r);
+        a.update_luciferin = 0;
+        a.guard_count = nullptr;
+        a.stats = st.p;
+        // the per-block partials are scratch: no epilogue, the run state stays as it was
+        HIPCHK(launch_score32(a, r->variant, r->stream_h));
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        HIPCHK(hipMemcpy(h.data(), st.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 6; i++) stats[i] = (int64_t)h[i];
     });
 }
 

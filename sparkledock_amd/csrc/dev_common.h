@@ -24,6 +24,13 @@ __device__ __forceinline__ float4 gld(const float4 *p, size_t i) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+// Two consecutive doubles p[2i], p[2i+1] in one 16-byte load.
+__device__ __forceinline__ double2 gld2(const double *p, size_t i) {
+    typedef double d2 __attribute__((ext_vector_type(2)));
+    const d2 v = ((const __attribute__((address_space(1))) d2 *)p)[i];
+    return make_double2(v.x, v.y);
+}
+
 // inverse = conj(q) / |q|^2 (prepare_gso_tasks.cc:288-302)
 __device__ __forceinline__ void quat_inverse(const double q[4], double inv[4]) {
     const double n2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3];

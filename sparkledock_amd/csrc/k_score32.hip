@@ -122,6 +122,7 @@ __device__ __forceinline__ float d2_fixed(int4 a, int4 b) {
 // Per-wave state shared by both filter variants.
 struct WaveCtx {
     int lane, gid, b, nblk, ncand;
+    int t_scan;       // next tile to scan for candidates
     const double *pose;
     double cb[3];     // block centre (fixed-point origin)
     float fc[3];
@@ -130,10 +131,36 @@ struct WaveCtx {
     int lig;          // its original index (-1 = padding)
 };
 
-// Set-up common to both variants: pose, block sphere, ligand atom, receptor
+// Candidate tiles from tile t0 on: 64 sphere tests per instruction, listed
+// in LDS (at most `cap` entries); advances t0 past the tiles tested.
+__device__ __forceinline__ int scan_candidates(const DevComplex &c, const WaveCtx &x,
+                                               unsigned short *ct, int &t0, int cap) {
+    int n = 0;
+    for (; t0 < c.ntiles && n <= cap - 64; t0 += 64) {
+        const int t = t0 + x.lane;
+        bool ok = false;
+        if (t < c.ntiles) {
+            const float4 tl = gld(c.tiles, t);
+            const float tx = tl.x - x.fc[0], ty = tl.y - x.fc[1], tz = tl.z - x.fc[2];
+            const float lim = x.reach + tl.w;
+            ok = tx * tx + ty * ty + tz * tz <= lim * lim;
+        }
+        const unsigned long long m = __ballot(ok);
+        if (ok)
+            ct[n + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                (unsigned short)t;
+        n += __popcll(m);
+    }
+    wave_sync_lds();
+    return n;
+}
+
+// Set-up common to the variants: pose, block sphere, ligand atom, receptor
 // extents (LDS), candidate tiles (LDS).  Returns false for an idle wave.
 __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex &c, int gid,
-                                           int b, float *ef, unsigned short *ct, WaveCtx &x) {
+                                           int b, float *ef, unsigned short *ct, WaveCtx &x,
+                                           int cap = kMaxTiles) {
     x.lane = threadIdx.x & 63;
     x.gid = gid;
     x.b = b;
@@ -170,26 +197,8 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     x.fc[0] = (float)x.cb[0];
     x.fc[1] = (float)x.cb[1];
     x.fc[2] = (float)x.cb[2];
-    // candidate tiles: 64 sphere tests per instruction, listed in LDS
-    int n = 0;
-    for (int t0 = 0; t0 < c.ntiles; t0 += 64) {
-        const int t = t0 + x.lane;
-        bool ok = false;
-        if (t < c.ntiles) {
-            const float4 tl = gld(c.tiles, t);
-            const float tx = tl.x - x.fc[0], ty = tl.y - x.fc[1], tz = tl.z - x.fc[2];
-            const float lim = x.reach + tl.w;
-            ok = tx * tx + ty * ty + tz * tz <= lim * lim;
-        }
-        const unsigned long long m = __ballot(ok);
-        if (ok)
-            ct[n + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
-                (unsigned short)t;
-        n += __popcll(m);
-    }
-    x.ncand = n;
-    wave_sync_lds();
+    x.t_scan = 0;
+    x.ncand = scan_candidates(c, x, ct, x.t_scan, cap);
     return true;
 }
 
@@ -241,7 +250,8 @@ __device__ __forceinline__ bool tile_atom(const DevComplex &c, const WaveCtx &x,
 
 __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
                                                const Kahan &acc, int npairs,
-                                               unsigned int nguard) {
+                                               unsigned int nguard, long long n_near,
+                                               long long n_queued) {
     double e = (double)acc.sum - (double)acc.comp;
     long long np = npairs;
     for (int off = 32; off > 0; off >>= 1) {
@@ -254,6 +264,14 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
         a.partial[(size_t)x.gid * a.max_lblocks + x.b] = e;
         a.partial_pairs[(size_t)x.gid * a.max_lblocks + x.b] = np;
         if (a.guard_count && ng) atomicAdd(a.guard_count, (unsigned long long)ng);
+        if (a.stats) {
+            atomicAdd(a.stats + 0, 1ull);
+            atomicAdd(a.stats + 1, x.ncand ? 1ull : 0ull);
+            atomicAdd(a.stats + 2, (unsigned long long)x.ncand);
+            atomicAdd(a.stats + 3, (unsigned long long)n_near);
+            atomicAdd(a.stats + 4, (unsigned long long)n_queued);
+            atomicAdd(a.stats + 5, (unsigned long long)np);
+        }
     }
 }
 
@@ -287,8 +305,10 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
     int npairs = 0;
     unsigned int nguard = 0;
     int qn = 0;
+    long long n_near = 0, n_queued = 0;
     auto flush = [&]() {
         wave_sync_lds();
+        n_queued += qn;
         for (int base = 0; base < qn; base += 64 * kUnroll) {
             float v[kUnroll];
             bool take[kUnroll], near[kUnroll];
@@ -348,6 +368,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
         int4 R;
         const bool nearb = tile_atom(c, x, ef, t, tl, rx, ry, rz, R);
         unsigned long long mask = __ballot(nearb);
+        n_near += __popcll(mask);
         const int rorig = nearb ? gld(c.rperm, t * kTile + lane) : 0;
         const int tag_l = lane << 24;
         while (mask) {
@@ -375,7 +396,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
         }
     }
     if (qn) flush();
-    write_partials(a, x, acc, npairs, nguard);
+    write_partials(a, x, acc, npairs, nguard, n_near, n_queued);
 }
 
 // ---------------------------------------------------------------- mfma variant
@@ -424,8 +445,10 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
     int npairs = 0;
     unsigned int nguard = 0;
     int qn = 0;
+    long long n_near = 0, n_queued = 0;
     auto flush = [&]() {
         wave_sync_lds();
+        n_queued += qn;
         for (int base = 0; base < qn; base += 64 * kUnroll) {
             float v[kUnroll];
             bool take[kUnroll], near[kUnroll];
@@ -486,7 +509,8 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
         const bool nearb = tile_atom(c, x, ef, t, tl, rx, ry, rz, R);
         const unsigned long long nmask = __ballot(nearb);
         if (!nmask) continue;
-        const int n_near = __popcll(nmask);
+        const int n_near_t = __popcll(nmask);
+        n_near += n_near_t;
         if (nearb) {
             const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(nmask >> 32),
                                                      __builtin_amdgcn_mbcnt_lo((unsigned)nmask, 0u));
@@ -494,10 +518,10 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
             ro[pos] = gld(c.rperm, t * kTile + lane);
         }
         wave_sync_lds();
-        for (int h = 0; h * 32 < n_near; h++) {
+        for (int h = 0; h * 32 < n_near_t; h++) {
             // A operands of 32 packed receptor atoms: r' = (x, y, z, |r|^2, 1, 0)
             const int rc = 32 * h + (lane & 31);
-            const int4 q = rc < n_near ? rt[rc] : make_int4(kFar, kFar, kFar, 0);
+            const int4 q = rc < n_near_t ? rt[rc] : make_int4(kFar, kFar, kFar, 0);
             const float px = (float)q.x * kInvQ, py = (float)q.y * kInvQ, pz = (float)q.z * kInvQ;
             const float r2 = px * px + py * py + pz * pz;
             const float a0 = lo_half ? px : py;
@@ -534,12 +558,235 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
                 }
                 // flush when the next group might not fit, and before the packed
                 // tile in LDS is replaced by the next tile
-                const bool last = 32 * (h + 1) >= n_near && 32 * (g + 1) >= x.nblk;
+                const bool last = 32 * (h + 1) >= n_near_t && 32 * (g + 1) >= x.nblk;
                 if (qn > kQueueM - 1024 || (last && qn)) flush();
             }
         }
     }
-    write_partials(a, x, acc, npairs, nguard);
+    write_partials(a, x, acc, npairs, nguard, n_near, n_queued);
+}
+
+// ---------------------------------------------------------------- broadcast variant
+// Near receptor atoms are collected across the visited tiles (their slots in a
+// pending list) and displaced 64 at a time, one per lane, so every lane of
+// the displacement has an atom; the displaced atoms are packed into LDS (two
+// batches' worth, alternating) and the wave walks them with broadcast LDS
+// reads, four per round.  Each lane pre-tests its ligand atom with the FP32
+// form V = (|r|^2 + |l|^2) - 2 r.l in A^2, and the pairs that pass are queued
+// as one word (packed receptor, ligand lane); the exact fixed-point d2 is
+// formed in the evaluate phase, where every lane has a pair.
+//
+// Pre-test bound: with u = 2^-24, every pair that can count (d2 < 225 + band)
+// has |r|, |l| <= rho, the wave's reach; converting the fixed-point values to
+// float, squaring and the four rounded adds of V stay within 28 u rho^2 of the
+// exact d2, so V <= 225 + 2 band + 2^-18 rho^2 + 10^-3 keeps every such pair
+// (extra pairs are rejected by the exact evaluation).
+constexpr int kQueueB = 768;
+constexpr int kCandB = 256;  // candidate-list chunk of the broadcast variant
+
+template <int kUnroll, int kMinWaves>
+__global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(ScoreArgs a) {
+    __shared__ int4 s_rt[kWaves][2 * 64];      // packed near receptor atoms: fixed point + row
+    __shared__ float4 s_rf[kWaves][2 * 64];    //   the same in A: x, y, z, |r|^2
+    __shared__ unsigned short s_rs[kWaves][2 * 64];  //   their receptor slots (tile*64 + atom)
+    __shared__ unsigned short s_pd[kWaves][2 * 64];  // near slots waiting for displacement
+    __shared__ int4 s_lg[kWaves][64];          // the block's ligand atoms: fixed point + type*20
+    __shared__ int s_q[kWaves][kQueueB];       // queued pairs: (packed receptor << 6) | lane
+    __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
+    __shared__ unsigned short s_ct[kWaves][kCandB];  // candidate tiles, scanned in chunks
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = blockIdx.x * kWaves + w;
+    if (wv >= a.n_work) return;
+    const int gid = a.work[2 * wv];
+    const int b = a.work[2 * wv + 1];
+    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
+    int4 *rt = s_rt[w];
+    float4 *rf = s_rf[w];
+    unsigned short *rs = s_rs[w];
+    unsigned short *pd = s_pd[w];
+    int4 *lg = s_lg[w];
+    int *qq = s_q[w];
+    float *ef = s_ef[w];
+    unsigned short *ct = s_ct[w];
+    WaveCtx x;
+    wave_setup(a, c, gid, b, ef, ct, x, kCandB);
+    const int lane = x.lane;
+    lg[lane] = x.L;
+    const float bandq = a.band * kQ2;
+    const float thr = 225.0f + 2.0f * a.band + 0x1p-18f * x.reach * x.reach + 1e-3f;
+    const float reach2 = x.reach * x.reach;
+    // this lane's pre-test operands: -2 l and |l|^2 (padding lanes never pass)
+    const float lx = (float)x.L.x * kInvQ, ly = (float)x.L.y * kInvQ, lz = (float)x.L.z * kInvQ;
+    const float m2x = -2.f * lx, m2y = -2.f * ly, m2z = -2.f * lz;
+    const float nl = x.lig >= 0 ? __builtin_fmaf(lz, lz, __builtin_fmaf(ly, ly, lx * lx)) : 1e30f;
+
+    Kahan acc;
+    int npairs = 0;
+    unsigned int nguard = 0;
+    int qn = 0;
+    unsigned int qbufs = 0;  // packed-batch buffers that queued pairs refer to
+    long long n_near = 0, n_queued = 0;
+    auto flush = [&]() {
+        wave_sync_lds();
+        n_queued += qn;
+        for (int base = 0; base < qn; base += 64 * kUnroll) {
+            float v[kUnroll];
+            bool take[kUnroll], near[kUnroll];
+            int tag[kUnroll];
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                const int e = base + u * 64 + lane;
+                const bool valid = e < qn;
+                const int raw = qq[min(e, kQueueB - 1)];
+                tag[u] = valid ? raw : 0;
+                const int4 r = rt[tag[u] >> 6];
+                const int4 l = lg[tag[u] & 63];
+                const PairClass pc = classify(d2_fixed(l, r), bandq);
+                near[u] = valid && pc.near;
+                take[u] = valid && pc.take;
+                v[u] = gld(c.table, take[u] ? r.w + l.w + pc.bin : kZeroIdx);
+            }
+#pragma unroll
+            for (int u = 0; u < kUnroll; u++) {
+                acc.add(v[u], take[u]);
+                npairs += take[u] ? 1 : 0;
+                if (near[u]) {
+                    const float2 g2 = guard_pair64(c, x.pose,
+                                                   gld(c.lperm, x.b * kBlock + (tag[u] & 63)),
+                                                   gld(c.rperm, (int)rs[tag[u] >> 6]));
+                    acc.add(g2.x, g2.y != 0.f);
+                    npairs += g2.y != 0.f ? 1 : 0;
+                    nguard++;
+                }
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        qn = 0;
+        qbufs = 0;
+    };
+
+    // Displace the pending atoms pd[0..n) (n <= 64, one per lane), pack them
+    // into buffer `buf` and queue their pairs that pass the pre-test.
+    int buf = 0;
+    auto batch = [&](int n) {
+        if (qbufs & (1u << buf)) flush();  // queued pairs still refer to this buffer
+        if (lane < n) {
+            const int slot = pd[lane];
+            const float4 tl = gld(c.tiles, slot / kTile);
+            const double2 rxy = gld2(c.rrel4, 2 * (size_t)slot);
+            const double rz = gld(c.rrel4, 4 * (size_t)slot + 2);
+            float Dx = 0.f, Dy = 0.f, Dz = 0.f;
+            const float4 *MA = (const float4 *)c.rmodA + (size_t)slot * (c.rmod_stride / 4);
+            for (int m0 = 0; m0 < c.nmr; m0 += 4) {
+                // four modes: twelve floats in three loads, added in mode order
+                const float4 q0 = gld(MA, 3 * (m0 / 4)), q1 = gld(MA, 3 * (m0 / 4) + 1),
+                             q2 = gld(MA, 3 * (m0 / 4) + 2);
+                const float mv[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y,
+                                      q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+#pragma unroll
+                for (int k = 0; k < 4; k++)
+                    if (m0 + k < c.nmr) {
+                        Dx = __builtin_fmaf(mv[3 * k], ef[m0 + k], Dx);
+                        Dy = __builtin_fmaf(mv[3 * k + 1], ef[m0 + k], Dy);
+                        Dz = __builtin_fmaf(mv[3 * k + 2], ef[m0 + k], Dz);
+                    }
+            }
+            const int4 R = make_int4(qround(((double)tl.x - x.cb[0]) + rxy.x + (double)Dx),
+                                     qround(((double)tl.y - x.cb[1]) + rxy.y + (double)Dy),
+                                     qround(((double)tl.z - x.cb[2]) + rz + (double)Dz),
+                                     gld(c.rtype_s, slot));
+            const float fx = (float)R.x * kInvQ, fy = (float)R.y * kInvQ, fz = (float)R.z * kInvQ;
+            const int p = 64 * buf + lane;
+            rt[p] = R;
+            rf[p] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
+            rs[p] = (unsigned short)slot;
+        }
+        wave_sync_lds();
+        // four packed atoms per round: their broadcast reads are in flight together
+        const int tag_b = (64 * buf) << 6;
+        for (int i0 = 0; i0 < n; i0 += 4) {
+            float4 r[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) r[j] = rf[64 * buf + min(i0 + j, 63)];
+            unsigned long long m[4];
+            bool in[4];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float d2a = __builtin_fmaf(
+                    r[j].x, m2x,
+                    __builtin_fmaf(r[j].y, m2y, __builtin_fmaf(r[j].z, m2z, r[j].w + nl)));
+                in[j] = i0 + j < n && d2a <= thr;
+                m[j] = __ballot(in[j]);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (m[j]) {
+                    if (in[j])
+                        qq[qn + __builtin_amdgcn_mbcnt_hi(
+                                    (unsigned)(m[j] >> 32),
+                                    __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
+                            tag_b | ((i0 + j) << 6) | lane;
+                    qn += __popcll(m[j]);
+                    qbufs |= 1u << buf;
+                }
+            }
+            if (qn > kQueueB - 4 * 64) flush();
+        }
+        buf ^= 1;
+    };
+
+    // Visit the candidate tiles (chunks of kCandB): near test of every slot
+    // against the block sphere; near slots join the pending list.
+    int npd = 0;
+    for (;;) {
+        int t_next = 0;
+        float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f), p_next = tl_next;
+        if (x.ncand) {
+            t_next = __builtin_amdgcn_readfirstlane((int)ct[0]);
+            tl_next = gld(c.tiles, t_next);
+            p_next = gld(c.rpos32, t_next * kTile + lane);
+        }
+        for (int k = 0; k < x.ncand; k++) {
+            const int t = t_next;
+            const float4 tl = tl_next, p = p_next;
+            if (k + 1 < x.ncand) {
+                t_next = __builtin_amdgcn_readfirstlane((int)ct[k + 1]);
+                tl_next = gld(c.tiles, t_next);
+                p_next = gld(c.rpos32, t_next * kTile + lane);
+            }
+            const float ax = p.x + (tl.x - x.fc[0]);
+            const float ay = p.y + (tl.y - x.fc[1]);
+            const float az = p.z + (tl.z - x.fc[2]);
+            const bool nearb = ax * ax + ay * ay + az * az <= reach2;
+            const unsigned long long m = __ballot(nearb);
+            if (!m) continue;
+            if (nearb)
+                pd[npd + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                    (unsigned short)(t * kTile + lane);
+            npd += __popcll(m);
+            n_near += __popcll(m);
+            if (npd >= 64) {
+                wave_sync_lds();
+                batch(64);
+                // move the remaining pending slots to the front
+                const int rem = npd - 64;
+                const unsigned short v = lane < rem ? pd[64 + lane] : (unsigned short)0;
+                wave_sync_lds();
+                if (lane < rem) pd[lane] = v;
+                npd = rem;
+            }
+        }
+        if (x.t_scan >= c.ntiles) break;
+        __builtin_amdgcn_wave_barrier();  // every lane has read the list before it is rewritten
+        x.ncand = scan_candidates(c, x, ct, x.t_scan, kCandB);
+    }
+    if (npd) {
+        wave_sync_lds();
+        batch(npd);
+    }
+    if (qn) flush();
+    write_partials(a, x, acc, npairs, nguard, n_near, n_queued);
 }
 
 // Per glowworm: energy = partials in block order (FP64), score, luciferin,
@@ -589,6 +836,9 @@ const Variant kVariants[] = {
     {"readlane/unroll4/occ3", score32r_kernel<4, 3>},
     {"mfma/unroll2/occ4", score32m_kernel<2, 4>},
     {"mfma/unroll4/occ3", score32m_kernel<4, 3>},
+    {"bcast/unroll4/occ4", score32b_kernel<4, 4>},
+    {"bcast/unroll2/occ5", score32b_kernel<2, 5>},
+    {"bcast/unroll4/occ3", score32b_kernel<4, 3>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 }  // namespace
@@ -602,9 +852,21 @@ const char *score32_variant_name(int v) {
 hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     if (a.n_work <= 0) return hipSuccess;
     if (variant < 0 || variant >= kNumVariants) variant = 0;
-    const int grid = (a.n_work + kWaves - 1) / kWaves;
-    hipLaunchKernelGGL(kVariants[variant].kernel, dim3(grid), dim3(kScore32Threads), 0, s, a);
-    return hipGetLastError();
+    // A launch holds fewer than 2^32 threads (2^26 waves); longer work lists
+    // are cut into chunks (the kernels address partials by glowworm and
+    // block, so a chunk only needs the offset work pointer).
+    constexpr int kChunk = (1 << 26) - kWaves;
+    for (int base = 0; base < a.n_work; base += kChunk) {
+        ScoreArgs part = a;
+        part.work = a.work + 2 * (size_t)base;
+        part.n_work = min(kChunk, a.n_work - base);
+        const int grid = (part.n_work + kWaves - 1) / kWaves;
+        hipLaunchKernelGGL(kVariants[variant].kernel, dim3(grid), dim3(kScore32Threads), 0, s,
+                           part);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s) {

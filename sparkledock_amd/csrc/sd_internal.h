@@ -55,7 +55,14 @@ struct DevComplex {
     int ntiles;
     const float4 *tiles;
     const double *rrel;      // [3][ntiles*64] coordinates - tile centre (f64)
-    const float *rmod32;
+    const float *rmod32;     // [nmr][3][ntiles*64] modes (f32)
+    // the same per slot, for gathers by slot: near-test coordinates (f32,
+    // w = 0), coordinates (f64 x, y, z, 0) and modes (f32, mode-major x, y, z,
+    // padded to a multiple of 4 floats: rmod_stride)
+    const float4 *rpos32;
+    const double *rrel4;
+    const float *rmodA;
+    int rmod_stride;
     const int *rtype_s;
     const int *rperm;
     float rmode_max[SDGSO_MAX_MODES];  // max over atoms of |M_m,i|
@@ -88,6 +95,9 @@ struct ScoreArgs {
     int max_lblocks;
     const int *work;        // FP32 form: work list of (glowworm, ligand block), or null
     int n_work;
+    // FP32 form, optional: filter statistics [waves, waves with candidate
+    // tiles, candidate tiles, near receptor atoms, queued pairs, in-cutoff pairs]
+    unsigned long long *stats;
 };
 
 struct SelectMoveArgs {

@@ -68,6 +68,8 @@ SIGNATURES = [
     ("sdgso_tune_variants", ctypes.c_int, []),
     ("sdgso_tune_variant_name", ctypes.c_char_p, [ctypes.c_int]),
     ("sdgso_run_tune", ctypes.c_int, [_VP, ctypes.c_int, _D, _I]),
+    ("sdgso_run_filter_stats", ctypes.c_int, [_VP, _I64]),
+    ("sdgso_run_set_variant", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_stream", ctypes.c_int,
      [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     ("sdgso_synth_molecule", ctypes.c_int,
@@ -258,10 +260,28 @@ class Run:
         names = [lib().sdgso_tune_variant_name(v).decode() for v in range(nv)]
         return ch.value, dict(zip(names, t.tolist()))
 
+    def set_variant(self, variant):
+        _check(lib().sdgso_run_set_variant(self.h, int(variant)))
+
     def guard_stats(self):
         s = np.zeros(1, np.int64)
         _check(lib().sdgso_run_guard_stats(self.h, _p(s, _I64)))
         return dict(recomputed=int(s[0]))
+
+    def filter_stats(self):
+        """Counts of one FP32 scoring pass over the current poses: waves
+        (glowworm x ligand block), waves with candidate tiles, candidate tiles,
+        receptor atoms passing the atom test, queued pairs, in-cutoff pairs."""
+        s = np.zeros(6, np.int64)
+        _check(lib().sdgso_run_filter_stats(self.h, _p(s, _I64)))
+        keys = ("waves", "waves_with_candidates", "candidate_tiles", "near_atoms", "queued_pairs",
+                "cutoff_pairs")
+        return dict(zip(keys, (int(v) for v in s)))
+
+
+def tune_variants():
+    """Names of the compiled FP32 scoring-kernel variants, by index."""
+    return [lib().sdgso_tune_variant_name(v).decode() for v in range(lib().sdgso_tune_variants())]
 
 
 def stream(seed, S, n_steps, G):

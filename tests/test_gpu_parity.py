@@ -23,6 +23,21 @@ def poses_of(case):
     return case["pos"].reshape(-1, case["pos"].shape[-1])
 
 
+VARIANTS = list(range(len(sdgso.tune_variants())))
+
+
+def score32(cx, poses, variant):
+    """FP32 guard-band scores and pair counts of `poses` with one kernel
+    variant: a run of single-glowworm swarms, scoring phase only."""
+    n = len(poses)
+    run = sdgso.Run(cx, n, 1, poses, None, 0, form=sdgso.FP32_GUARD)
+    run.set_variant(variant)
+    run.phases(1)
+    st = run.read()
+    run.close()
+    return st["score"], st["pairs"]
+
+
 @pytest.mark.parametrize("seed,nmr,nml", [(1, 10, 10), (2, 0, 0), (3, 4, 7)])
 def test_score_fp64_bit_exact(seed, nmr, nml):
     c = GC.case(seed, 500, 260, nmr, nml, 4, 16)
@@ -45,19 +60,28 @@ def test_score_fp32_guard(seed, nmr, nml):
     s0, ref0, pairs0 = pr.oracle_scores(poses)
     assert np.array_equal(pairs, pairs0)
     assert np.array_equal(ref, ref0)
-    e, e0 = (4.7 - s) / 0.0157, (4.7 - s0) / 0.0157
     denom = GC.sum_abs_terms(pr, poses)
-    assert np.all(np.abs(e - e0) <= REL_E * np.maximum(denom, 1.0))
+    e0 = (4.7 - s0) / 0.0157
+    for v in VARIANTS:
+        if v:
+            s, pairs = score32(pr.gpu, poses, v)
+            assert np.array_equal(pairs, pairs0), v
+        e = (4.7 - s) / 0.0157
+        assert np.all(np.abs(e - e0) <= REL_E * np.maximum(denom, 1.0)), v
 
 
-def per_bin_counts(pr, poses, form):
-    """Pair count of every bin through one-hot tables (exact integer sums)."""
+def per_bin_counts(pr, poses, form, variant=None):
+    """Pair count of every bin through one-hot tables (exact integer sums);
+    `variant` picks an FP32 kernel variant (default: the plugin call)."""
     out = []
     for b in range(20):
         t = np.zeros((168, 168, 20), np.float32)
         t[:, :, b] = 1.0
         cx = pr.gpu_with_table(t.reshape(-1))
-        s, _, _ = cx.score(poses, form)
+        if variant is None:
+            s, _, _ = cx.score(poses, form)
+        else:
+            s, _ = score32(cx, poses, variant)
         out.append(np.rint((4.7 - s) / 0.0157))
         cx.close()
     return np.array(out)
@@ -79,6 +103,8 @@ def test_bins_exact_on_lattice_boundaries():
     for form in (sdgso.FP64, sdgso.FP32_GUARD):
         got = per_bin_counts(pr, poses, form)
         assert np.array_equal(got, want), form
+    for v in VARIANTS[1:]:
+        assert np.array_equal(per_bin_counts(pr, poses, None, v), want), v
     # the full table too (FP64: bit-exact; FP32: same pairs)
     s0, _, p0 = pr.oracle_scores(poses)
     s, _, p = pr.gpu.score(poses, sdgso.FP64)
@@ -129,6 +155,10 @@ def test_fp32_guard_sweep_across_every_boundary():
         want.append(round((4.7 - pr.with_table(t.reshape(-1)).score_pose(poses[0])[0]) / 0.0157))
     got = per_bin_counts(pr, poses, sdgso.FP32_GUARD)[:, 0]
     assert np.array_equal(got, np.array(want))
+    for v in VARIANTS[1:]:
+        s, p = score32(pr.gpu, poses, v)
+        assert np.array_equal(p, p0) and abs(s[0] - s0[0]) < 1e-5, v
+        assert np.array_equal(per_bin_counts(pr, poses, None, v)[:, 0], np.array(want)), v
 
 
 def test_bins_exact_random_fp32():
@@ -139,6 +169,8 @@ def test_bins_exact_random_fp32():
     b = per_bin_counts(pr, poses, sdgso.FP32_GUARD)
     assert np.array_equal(a, b)
     assert a.sum() > 10000
+    for v in VARIANTS[1:]:
+        assert np.array_equal(per_bin_counts(pr, poses, None, v), a), v
 
 
 def make_run(pr, c, steps, form=sdgso.FP64, stream=None):
@@ -257,3 +289,62 @@ def test_step_past_stream_is_an_error():
     with pytest.raises(sdgso.SdgsoError) as e:
         run.steps(1)
     assert e.value.code == 4
+
+
+def test_filter_stats_consistent_with_pairs():
+    """The filter counters of a scoring pass agree with the exact pair counts
+    of the same poses, for every kernel variant; the pass changes no state."""
+    c = GC.case(16, 900, 260, 10, 10, 3, 24, swarm_radius=3.0)
+    pr = GC.Pair(c)
+    _, _, pairs64 = pr.gpu.score(poses_of(c), sdgso.FP64)
+    run, _ = make_run(pr, c, 2, form=sdgso.FP32_GUARD)
+    before = run.read()
+    n_blocks = -(-260 // 64)
+    for v in range(len(sdgso.tune_variants())):
+        run.set_variant(v)
+        st = run.filter_stats()
+        assert st["waves"] == 3 * 24 * n_blocks
+        assert 0 < st["waves_with_candidates"] <= st["waves"]
+        assert st["near_atoms"] <= 64 * st["candidate_tiles"]
+        assert st["cutoff_pairs"] == int(pairs64.sum())
+        assert st["cutoff_pairs"] <= st["queued_pairs"] <= 64 * st["near_atoms"]
+    after = run.read()
+    for key in before:
+        assert np.array_equal(before[key], after[key]), key
+
+
+def dense_case(seed=18, nr=20000, nl=64, n_poses=3):
+    """A receptor packed into a 12 A ball, so that every one of its 313 tiles
+    is a candidate of the ligand block (more than one candidate-list chunk)."""
+    rng = np.random.default_rng(seed)
+    r = rng.normal(size=(nr, 3))
+    r *= (12.0 * rng.random(nr) ** (1 / 3) / np.linalg.norm(r, axis=1))[:, None]
+    lg = rng.normal(size=(nl, 3)) * 2.0
+    c = dict(r_xyz=r, l_xyz=lg, r_type=rng.integers(0, 168, nr).astype(np.int32),
+             l_type=rng.integers(0, 168, nl).astype(np.int32),
+             table=(rng.normal(size=168 * 168 * 20) * 0.5).astype(np.float32),
+             r_modes=np.zeros((0, nr, 3)), l_modes=np.zeros((0, nl, 3)),
+             r_mask=np.ones(nr, np.int32), l_mask=np.ones(nl, np.int32), lig_ref=np.zeros(3))
+    q = rng.normal(size=(n_poses, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    t = rng.normal(size=(n_poses, 3)) * 1.0
+    c["pos"] = np.concatenate([t, q], axis=1).reshape(1, n_poses, 7)
+    return c
+
+
+def test_every_tile_a_candidate():
+    c = dense_case()
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, p0 = pr.oracle_scores(poses)
+    s64, _, p64 = pr.gpu.score(poses, sdgso.FP64)
+    assert np.array_equal(p64, p0) and np.array_equal(s64, s0)
+    denom = GC.sum_abs_terms(pr, poses)
+    e0 = (4.7 - s0) / 0.0157
+    for v in VARIANTS:
+        s, p = score32(pr.gpu, poses, v)
+        assert np.array_equal(p, p0), v
+        assert np.all(np.abs((4.7 - s) / 0.0157 - e0) <= REL_E * np.maximum(denom, 1.0)), v
+    run = sdgso.Run(pr.gpu, 1, len(poses), poses, None, 0, form=sdgso.FP32_GUARD)
+    st = run.filter_stats()
+    assert st["candidate_tiles"] == len(poses) * 313
