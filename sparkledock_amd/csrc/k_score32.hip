@@ -250,8 +250,8 @@ __device__ __forceinline__ bool tile_atom(const DevComplex &c, const WaveCtx &x,
 
 __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
                                                const Kahan &acc, int npairs,
-                                               unsigned int nguard, long long n_near,
-                                               long long n_queued) {
+                                               unsigned int nguard, long long n_cand,
+                                               long long n_near, long long n_queued) {
     double e = (double)acc.sum - (double)acc.comp;
     long long np = npairs;
     for (int off = 32; off > 0; off >>= 1) {
@@ -266,8 +266,8 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
         if (a.guard_count && ng) atomicAdd(a.guard_count, (unsigned long long)ng);
         if (a.stats) {
             atomicAdd(a.stats + 0, 1ull);
-            atomicAdd(a.stats + 1, x.ncand ? 1ull : 0ull);
-            atomicAdd(a.stats + 2, (unsigned long long)x.ncand);
+            atomicAdd(a.stats + 1, n_cand ? 1ull : 0ull);
+            atomicAdd(a.stats + 2, (unsigned long long)n_cand);
             atomicAdd(a.stats + 3, (unsigned long long)n_near);
             atomicAdd(a.stats + 4, (unsigned long long)n_queued);
             atomicAdd(a.stats + 5, (unsigned long long)np);
@@ -396,7 +396,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
         }
     }
     if (qn) flush();
-    write_partials(a, x, acc, npairs, nguard, n_near, n_queued);
+    write_partials(a, x, acc, npairs, nguard, x.ncand, n_near, n_queued);
 }
 
 // ---------------------------------------------------------------- mfma variant
@@ -563,15 +563,14 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
             }
         }
     }
-    write_partials(a, x, acc, npairs, nguard, n_near, n_queued);
+    write_partials(a, x, acc, npairs, nguard, x.ncand, n_near, n_queued);
 }
 
 // ---------------------------------------------------------------- broadcast variant
 // Near receptor atoms are collected across the visited tiles (their slots in a
 // pending list) and displaced 64 at a time, one per lane, so every lane of
-// the displacement has an atom; the displaced atoms are packed into LDS (two
-// batches' worth, alternating) and the wave walks them with broadcast LDS
-// reads, four per round.  Each lane pre-tests its ligand atom with the FP32
+// the displacement has an atom; the displaced atoms are packed into LDS and
+// the wave walks them with broadcast LDS reads, four per round.  Each lane pre-tests its ligand atom with the FP32
 // form V = (|r|^2 + |l|^2) - 2 r.l in A^2, and the pairs that pass are queued
 // as one word (packed receptor, ligand lane); the exact fixed-point d2 is
 // formed in the evaluate phase, where every lane has a pair.
@@ -581,14 +580,14 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 // float, squaring and the four rounded adds of V stay within 28 u rho^2 of the
 // exact d2, so V <= 225 + 2 band + 2^-18 rho^2 + 10^-3 keeps every such pair
 // (extra pairs are rejected by the exact evaluation).
-constexpr int kQueueB = 768;
-constexpr int kCandB = 256;  // candidate-list chunk of the broadcast variant
+constexpr int kQueueB = 512;
+constexpr int kCandB = 128;  // candidate-list chunk of the broadcast variant
 
 template <int kUnroll, int kMinWaves>
 __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(ScoreArgs a) {
-    __shared__ int4 s_rt[kWaves][2 * 64];      // packed near receptor atoms: fixed point + row
-    __shared__ float4 s_rf[kWaves][2 * 64];    //   the same in A: x, y, z, |r|^2
-    __shared__ unsigned short s_rs[kWaves][2 * 64];  //   their receptor slots (tile*64 + atom)
+    __shared__ int4 s_rt[kWaves][64];          // packed near receptor atoms: fixed point + row
+    __shared__ float4 s_rf[kWaves][64];        //   the same in A: x, y, z, |r|^2
+    __shared__ unsigned short s_rs[kWaves][64];  //   their receptor slots (tile*64 + atom)
     __shared__ unsigned short s_pd[kWaves][2 * 64];  // near slots waiting for displacement
     __shared__ int4 s_lg[kWaves][64];          // the block's ligand atoms: fixed point + type*20
     __shared__ int s_q[kWaves][kQueueB];       // queued pairs: (packed receptor << 6) | lane
@@ -624,8 +623,24 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
     int npairs = 0;
     unsigned int nguard = 0;
     int qn = 0;
-    unsigned int qbufs = 0;  // packed-batch buffers that queued pairs refer to
     long long n_near = 0, n_queued = 0;
+    // table values of the previous evaluate round, still in flight: they are
+    // added one round later (or at the end), so their latency overlaps work
+    float pv[kUnroll];
+    bool ptake[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+        pv[u] = 0.f;
+        ptake[u] = false;
+    }
+    auto drain = [&]() {
+#pragma unroll
+        for (int u = 0; u < kUnroll; u++) {
+            acc.add(pv[u], ptake[u]);
+            npairs += ptake[u] ? 1 : 0;
+            ptake[u] = false;
+        }
+    };
     auto flush = [&]() {
         wave_sync_lds();
         n_queued += qn;
@@ -646,10 +661,11 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                 take[u] = valid && pc.take;
                 v[u] = gld(c.table, take[u] ? r.w + l.w + pc.bin : kZeroIdx);
             }
+            drain();
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
-                acc.add(v[u], take[u]);
-                npairs += take[u] ? 1 : 0;
+                pv[u] = v[u];
+                ptake[u] = take[u];
                 if (near[u]) {
                     const float2 g2 = guard_pair64(c, x.pose,
                                                    gld(c.lperm, x.b * kBlock + (tag[u] & 63)),
@@ -662,14 +678,12 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
         }
         __builtin_amdgcn_wave_barrier();
         qn = 0;
-        qbufs = 0;
     };
 
     // Displace the pending atoms pd[0..n) (n <= 64, one per lane), pack them
-    // into buffer `buf` and queue their pairs that pass the pre-test.
-    int buf = 0;
+    // into LDS, queue their pairs that pass the pre-test and evaluate them
+    // (the queue is empty between batches: its entries index the packed atoms).
     auto batch = [&](int n) {
-        if (qbufs & (1u << buf)) flush();  // queued pairs still refer to this buffer
         if (lane < n) {
             const int slot = pd[lane];
             const float4 tl = gld(c.tiles, slot / kTile);
@@ -677,37 +691,45 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
             const double rz = gld(c.rrel4, 4 * (size_t)slot + 2);
             float Dx = 0.f, Dy = 0.f, Dz = 0.f;
             const float4 *MA = (const float4 *)c.rmodA + (size_t)slot * (c.rmod_stride / 4);
-            for (int m0 = 0; m0 < c.nmr; m0 += 4) {
-                // four modes: twelve floats in three loads, added in mode order
-                const float4 q0 = gld(MA, 3 * (m0 / 4)), q1 = gld(MA, 3 * (m0 / 4) + 1),
-                             q2 = gld(MA, 3 * (m0 / 4) + 2);
-                const float mv[12] = {q0.x, q0.y, q0.z, q0.w, q1.x, q1.y,
-                                      q1.z, q1.w, q2.x, q2.y, q2.z, q2.w};
+            const int ngroups = (c.nmr + 3) / 4;  // groups of four modes: twelve floats, three loads
+            for (int g0 = 0; g0 < ngroups; g0 += 3) {
+                // up to three groups per round, loads issued together (a group past
+                // the last one re-reads the last; its products are not added)
+                float4 q[9];
 #pragma unroll
-                for (int k = 0; k < 4; k++)
-                    if (m0 + k < c.nmr) {
-                        Dx = __builtin_fmaf(mv[3 * k], ef[m0 + k], Dx);
-                        Dy = __builtin_fmaf(mv[3 * k + 1], ef[m0 + k], Dy);
-                        Dz = __builtin_fmaf(mv[3 * k + 2], ef[m0 + k], Dz);
+                for (int j = 0; j < 3; j++)
+                    for (int h = 0; h < 3; h++) q[3 * j + h] = gld(MA, 3 * min(g0 + j, ngroups - 1) + h);
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    const float mv[12] = {q[3 * j].x,     q[3 * j].y,     q[3 * j].z,     q[3 * j].w,
+                                          q[3 * j + 1].x, q[3 * j + 1].y, q[3 * j + 1].z, q[3 * j + 1].w,
+                                          q[3 * j + 2].x, q[3 * j + 2].y, q[3 * j + 2].z, q[3 * j + 2].w};
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        const int m = 4 * (g0 + j) + k;
+                        if (m < c.nmr) {
+                            Dx = __builtin_fmaf(mv[3 * k], ef[m], Dx);
+                            Dy = __builtin_fmaf(mv[3 * k + 1], ef[m], Dy);
+                            Dz = __builtin_fmaf(mv[3 * k + 2], ef[m], Dz);
+                        }
                     }
+                }
             }
             const int4 R = make_int4(qround(((double)tl.x - x.cb[0]) + rxy.x + (double)Dx),
                                      qround(((double)tl.y - x.cb[1]) + rxy.y + (double)Dy),
                                      qround(((double)tl.z - x.cb[2]) + rz + (double)Dz),
                                      gld(c.rtype_s, slot));
             const float fx = (float)R.x * kInvQ, fy = (float)R.y * kInvQ, fz = (float)R.z * kInvQ;
-            const int p = 64 * buf + lane;
-            rt[p] = R;
-            rf[p] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
-            rs[p] = (unsigned short)slot;
+            rt[lane] = R;
+            rf[lane] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
+            rs[lane] = (unsigned short)slot;
         }
         wave_sync_lds();
         // four packed atoms per round: their broadcast reads are in flight together
-        const int tag_b = (64 * buf) << 6;
         for (int i0 = 0; i0 < n; i0 += 4) {
             float4 r[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) r[j] = rf[64 * buf + min(i0 + j, 63)];
+            for (int j = 0; j < 4; j++) r[j] = rf[min(i0 + j, 63)];
             unsigned long long m[4];
             bool in[4];
 #pragma unroll
@@ -715,8 +737,8 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                 const float d2a = __builtin_fmaf(
                     r[j].x, m2x,
                     __builtin_fmaf(r[j].y, m2y, __builtin_fmaf(r[j].z, m2z, r[j].w + nl)));
-                in[j] = i0 + j < n && d2a <= thr;
-                m[j] = __ballot(in[j]);
+                in[j] = d2a <= thr;
+                m[j] = i0 + j < n ? __ballot(in[j]) : 0ull;
             }
 #pragma unroll
             for (int j = 0; j < 4; j++) {
@@ -725,20 +747,21 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                         qq[qn + __builtin_amdgcn_mbcnt_hi(
                                     (unsigned)(m[j] >> 32),
                                     __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
-                            tag_b | ((i0 + j) << 6) | lane;
+                            ((i0 + j) << 6) | lane;
                     qn += __popcll(m[j]);
-                    qbufs |= 1u << buf;
                 }
             }
             if (qn > kQueueB - 4 * 64) flush();
         }
-        buf ^= 1;
+        if (qn) flush();
     };
 
     // Visit the candidate tiles (chunks of kCandB): near test of every slot
     // against the block sphere; near slots join the pending list.
     int npd = 0;
+    long long n_cand = 0;
     for (;;) {
+        n_cand += x.ncand;
         int t_next = 0;
         float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f), p_next = tl_next;
         if (x.ncand) {
@@ -785,8 +808,8 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
         wave_sync_lds();
         batch(npd);
     }
-    if (qn) flush();
-    write_partials(a, x, acc, npairs, nguard, n_near, n_queued);
+    drain();
+    write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued);
 }
 
 // Per glowworm: energy = partials in block order (FP64), score, luciferin,
@@ -831,14 +854,13 @@ struct Variant {
     void (*kernel)(ScoreArgs);
 };
 const Variant kVariants[] = {
-    {"readlane/unroll4/occ4", score32r_kernel<4, 4>},
-    {"readlane/unroll2/occ5", score32r_kernel<2, 5>},
-    {"readlane/unroll4/occ3", score32r_kernel<4, 3>},
-    {"mfma/unroll2/occ4", score32m_kernel<2, 4>},
-    {"mfma/unroll4/occ3", score32m_kernel<4, 3>},
-    {"bcast/unroll4/occ4", score32b_kernel<4, 4>},
     {"bcast/unroll2/occ5", score32b_kernel<2, 5>},
-    {"bcast/unroll4/occ3", score32b_kernel<4, 3>},
+    {"bcast/unroll2/occ6", score32b_kernel<2, 6>},
+    {"bcast/unroll4/occ5", score32b_kernel<4, 5>},
+    {"bcast/unroll4/occ4", score32b_kernel<4, 4>},
+    {"readlane/unroll4/occ4", score32r_kernel<4, 4>},
+    {"readlane/unroll4/occ3", score32r_kernel<4, 3>},
+    {"mfma/unroll4/occ3", score32m_kernel<4, 3>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 }  // namespace
