@@ -43,6 +43,7 @@
 namespace sd {
 
 typedef float floatx16 __attribute__((ext_vector_type(16)));
+typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 
 constexpr int kScore32Threads = 256;
 constexpr int kWaves = kScore32Threads / 64;
@@ -127,6 +128,7 @@ struct WaveCtx {
     double cb[3];     // block centre (fixed-point origin)
     float fc[3];
     float reach;      // cutoff + block radius + displacement bounds
+    float rmax;       // bound of |r| for the receptor atoms that pass the atom test
     int4 L;           // this lane's ligand atom: fixed point + type*20
     int lig;          // its original index (-1 = padding)
 };
@@ -194,6 +196,7 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     dr = dr * 1.0001f + 1e-3f;
     dl = dl * 1.0001f + 1e-3f;
     x.reach = 15.0f + blk.w + dl + dr + 0.01f;
+    x.rmax = x.reach + dr + 0.01f;
     x.fc[0] = (float)x.cb[0];
     x.fc[1] = (float)x.cb[1];
     x.fc[2] = (float)x.cb[2];
@@ -576,21 +579,35 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 // formed in the evaluate phase, where every lane has a pair.
 //
 // Pre-test bound: with u = 2^-24, every pair that can count (d2 < 225 + band)
-// has |r|, |l| <= rho, the wave's reach; converting the fixed-point values to
-// float, squaring and the four rounded adds of V stay within 28 u rho^2 of the
-// exact d2, so V <= 225 + 2 band + 2^-18 rho^2 + 10^-3 keeps every such pair
-// (extra pairs are rejected by the exact evaluation).
+// has |l| <= rho - 15 and |r| <= rho, rho the wave's reach; converting the
+// fixed-point values to float, squaring and the four rounded adds of V stay
+// within 28 u rho^2 of the exact d2, so V <= 225 + 2 band + 2^-18 rho^2 + 10^-3
+// keeps every such pair (extra pairs are rejected by the exact evaluation).
+//
+// MFMA pre-test (kMfma): V of 32 x 32 pairs is one v_mfma_f32_32x32x8f16 with
+// rows (x, y, z, n_hi | n_lo, 1, 1, 0) of the packed receptor atoms and
+// columns (-2x, -2y, -2z, 1 | 1, m_hi, m_lo, 0) of the ligand atoms, n and m
+// the squared norms split into two halves.  Half precision rounds each
+// coordinate within 2^-11 relative, so the cross term is within
+// 2^-9 |r||l| <= 2^-9 rho (rho - 15); the split norms and the FP32
+// accumulation add less than 2^-18 rho^2.  Pairs with
+// V <= 225 + 2 band + 2^-8 rho (rho - 15) + 2^-17 rho^2 + 10^-3 are queued.
+// Rows past the batch and padding ligand lanes carry a norm of 60000, so they
+// never pass; a wave whose atoms could exceed the half range (|r| >= 250 A)
+// takes the FP32 pre-test.
 constexpr int kQueueB = 512;
+constexpr int kQueueBM = 1280;  // MFMA pre-test: room for a 32 x 32 group over the flush mark
 constexpr int kCandB = 128;  // candidate-list chunk of the broadcast variant
 
-template <int kUnroll, int kMinWaves>
+template <int kUnroll, int kMinWaves, bool kMfma>
 __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(ScoreArgs a) {
+    constexpr int kQueue = kMfma ? kQueueBM : kQueueB;
     __shared__ int4 s_rt[kWaves][64];          // packed near receptor atoms: fixed point + row
     __shared__ float4 s_rf[kWaves][64];        //   the same in A: x, y, z, |r|^2
     __shared__ unsigned short s_rs[kWaves][64];  //   their receptor slots (tile*64 + atom)
     __shared__ unsigned short s_pd[kWaves][2 * 64];  // near slots waiting for displacement
     __shared__ int4 s_lg[kWaves][64];          // the block's ligand atoms: fixed point + type*20
-    __shared__ int s_q[kWaves][kQueueB];       // queued pairs: (packed receptor << 6) | lane
+    __shared__ int s_q[kWaves][kQueue];        // queued pairs: (packed receptor << 6) | lane
     __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
     __shared__ unsigned short s_ct[kWaves][kCandB];  // candidate tiles, scanned in chunks
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -611,6 +628,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
     wave_setup(a, c, gid, b, ef, ct, x, kCandB);
     const int lane = x.lane;
     lg[lane] = x.L;
+    wave_sync_lds();
     const float bandq = a.band * kQ2;
     const float thr = 225.0f + 2.0f * a.band + 0x1p-18f * x.reach * x.reach + 1e-3f;
     const float reach2 = x.reach * x.reach;
@@ -618,6 +636,29 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
     const float lx = (float)x.L.x * kInvQ, ly = (float)x.L.y * kInvQ, lz = (float)x.L.z * kInvQ;
     const float m2x = -2.f * lx, m2y = -2.f * ly, m2z = -2.f * lz;
     const float nl = x.lig >= 0 ? __builtin_fmaf(lz, lz, __builtin_fmaf(ly, ly, lx * lx)) : 1e30f;
+    // MFMA pre-test: B operands of the two 32-atom column groups, and the threshold
+    const bool mfma_ok = kMfma && x.rmax < 250.0f;
+    const float thr16 = 225.0f + 2.0f * a.band + 0x1p-8f * x.reach * (x.reach - 15.0f) +
+                        0x1p-17f * x.reach * x.reach + 1e-3f;
+    half4 bop[2];
+    if (kMfma) {
+#pragma unroll
+        for (int g = 0; g < 2; g++) {
+            const int j = 32 * g + (lane & 31);
+            const int4 q = lg[j];
+            const bool real = gld(c.lperm, x.b * kBlock + j) >= 0;
+            const float px = real ? (float)q.x * kInvQ : 0.f, py = real ? (float)q.y * kInvQ : 0.f,
+                        pz = real ? (float)q.z * kInvQ : 0.f;
+            const float m = real ? __builtin_fmaf(pz, pz, __builtin_fmaf(py, py, px * px)) : 60000.f;
+            const _Float16 mh = (_Float16)m;
+            const _Float16 ml = (_Float16)(m - (float)mh);
+            if (lane < 32)
+                bop[g] = half4{(_Float16)(-2.f * px), (_Float16)(-2.f * py), (_Float16)(-2.f * pz),
+                               (_Float16)1.f};
+            else
+                bop[g] = half4{(_Float16)1.f, mh, ml, (_Float16)0.f};
+        }
+    }
 
     Kahan acc;
     int npairs = 0;
@@ -652,7 +693,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
             for (int u = 0; u < kUnroll; u++) {
                 const int e = base + u * 64 + lane;
                 const bool valid = e < qn;
-                const int raw = qq[min(e, kQueueB - 1)];
+                const int raw = qq[min(e, kQueue - 1)];
                 tag[u] = valid ? raw : 0;
                 const int4 r = rt[tag[u] >> 6];
                 const int4 l = lg[tag[u] & 63];
@@ -725,6 +766,52 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
             rs[lane] = (unsigned short)slot;
         }
         wave_sync_lds();
+        if (mfma_ok) {
+            // lane l: row 8*(v/4) + 4*(l/32) + v%4 of accumulator v, column l%32
+            const int lanetag = ((4 * (lane >> 5)) << 6) | (lane & 31);
+            for (int h = 0; h * 32 < n; h++) {
+                const int ri = 32 * h + (lane & 31);
+                const float4 r = rf[ri];
+                half4 av;
+                if (ri < n) {
+                    const _Float16 nh = (_Float16)r.w;
+                    if (lane < 32)
+                        av = half4{(_Float16)r.x, (_Float16)r.y, (_Float16)r.z, nh};
+                    else
+                        av = half4{(_Float16)(r.w - (float)nh), (_Float16)1.f, (_Float16)1.f,
+                                   (_Float16)0.f};
+                } else {
+                    av = lane < 32 ? half4{(_Float16)0.f, (_Float16)0.f, (_Float16)0.f,
+                                           (_Float16)60000.f}
+                                   : half4{(_Float16)0.f, (_Float16)0.f, (_Float16)0.f,
+                                           (_Float16)0.f};
+                }
+#pragma unroll 1
+                for (int g = 0; g < 2; g++) {
+                    if (32 * g >= x.nblk) break;
+                    if (qn > kQueue - 1024) flush();
+                    floatx16 d = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
+                                  0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                    d = __builtin_amdgcn_mfma_f32_32x32x8f16(av, g ? bop[1] : bop[0], d, 0, 0, 0);
+                    const int tagb = ((32 * h) << 6) + 32 * g + lanetag;
+#pragma unroll
+                    for (int v = 0; v < 16; v++) {
+                        const bool hit = d[v] <= thr16;
+                        const unsigned long long m = __ballot(hit);
+                        if (m) {
+                            if (hit)
+                                qq[qn + __builtin_amdgcn_mbcnt_hi(
+                                            (unsigned)(m >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                                    tagb + ((8 * (v >> 2) + (v & 3)) << 6);
+                            qn += __popcll(m);
+                        }
+                    }
+                }
+            }
+            if (qn) flush();
+            return;
+        }
         // four packed atoms per round: their broadcast reads are in flight together
         for (int i0 = 0; i0 < n; i0 += 4) {
             float4 r[4];
@@ -751,7 +838,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                     qn += __popcll(m[j]);
                 }
             }
-            if (qn > kQueueB - 4 * 64) flush();
+            if (qn > kQueue - 4 * 64) flush();
         }
         if (qn) flush();
     };
@@ -854,10 +941,10 @@ struct Variant {
     void (*kernel)(ScoreArgs);
 };
 const Variant kVariants[] = {
-    {"bcast/unroll2/occ5", score32b_kernel<2, 5>},
-    {"bcast/unroll2/occ6", score32b_kernel<2, 6>},
-    {"bcast/unroll4/occ5", score32b_kernel<4, 5>},
-    {"bcast/unroll4/occ4", score32b_kernel<4, 4>},
+    {"bcast-mfma16/unroll4/occ4", score32b_kernel<4, 4, true>},
+    {"bcast-mfma16/unroll2/occ4", score32b_kernel<2, 4, true>},
+    {"bcast/unroll4/occ4", score32b_kernel<4, 4, false>},
+    {"bcast/unroll2/occ5", score32b_kernel<2, 5, false>},
     {"readlane/unroll4/occ4", score32r_kernel<4, 4>},
     {"readlane/unroll4/occ3", score32r_kernel<4, 3>},
     {"mfma/unroll4/occ3", score32m_kernel<4, 3>},
