@@ -57,6 +57,7 @@ def precision_check(wl, n_per_complex=64, seed=5):
     n = 0
     worst_rel = 0.0
     worst_rel_score = 0.0
+    rel_e = []
     tab_abs = np.abs(sdgso.synth_table(workloads.TABLE_SEED))
     for cx, S, inp in zip(wl["complexes"], wl["n_swarms"], wl["inputs"]):
         idx = start + rng.choice(S * G, size=min(n_per_complex, S * G), replace=False)
@@ -72,11 +73,14 @@ def precision_check(wl, n_per_complex=64, seed=5):
         worst_rel = max(worst_rel, float(np.max(np.abs(e32 - e64) / denom)))
         worst_rel_score = max(worst_rel_score,
                               float(np.max(np.abs(s32 - s64) / np.maximum(np.abs(s64), 1e-300))))
+        rel_e.append(np.abs(e32 - e64) / np.maximum(np.abs(e64), 1e-300))
         bins_equal += int(np.sum(p64 == p32))
         n += len(idx)
         start += S * G
+    rel_e = np.concatenate(rel_e)
     return dict(poses=n, pair_counts_equal=bins_equal, energy_rel_err_sum_abs=worst_rel,
-                score_rel_err=worst_rel_score)
+                energy_rel_err_max=float(rel_e.max()), energy_rel_err_median=float(np.median(rel_e)),
+                energy_rel_err_le_1e6=float(np.mean(rel_e <= 1e-6)), score_rel_err=worst_rel_score)
 
 
 def ranking_overlap(stream, workload="c1", steps=30):
@@ -159,6 +163,18 @@ def cpu_baseline(wl, threads, target_s=12.0):
                           t_total))
 
 
+def traffic_bytes(workload):
+    """HBM bytes per scoring launch from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE and WRITE_SIZE, calibrated), if any."""
+    p = os.path.join(ROOT, "profiles", "score32_%s_traffic.md" % workload)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("bytes per launch:"):
+            return int(line.split(":")[1].split()[0])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,10 +234,11 @@ def main():
     run.sync()
     if args.variant is None:
         chosen, tune_ms = run.tune()
-        log("autotune: variant %d, ms per launch %s" % (chosen, tune_ms))
+        log("autotune: variants per complex %s" % chosen)
+        log("autotune: ms per launch per complex %s" % tune_ms)
     else:
         run.set_variant(args.variant)
-        chosen, tune_ms = args.variant, None
+        chosen, tune_ms = run.variants(), None
     run.profile(True)
     dt = timed_steps(run, args.steps, graph=False)
     prof = run.profile_read()
@@ -276,6 +293,19 @@ def main():
 
     flops = 8.0 * pairs_per_launch
     achieved = flops / (score_ms * 1e-3) / 1e12
+    # the table reads: 4 bytes per in-cutoff pair against the measured random-gather rate
+    probe = sdgso.probe_gather()
+    gather_gbs = 4.0 * pairs_per_launch / (score_ms * 1e-3) / 1e9
+    names = sdgso.tune_variants()
+    chosen_names = [names[v] for v in chosen]
+    # the variant that scores the most work (waves) names the dominant kernel
+    blocks = [S * wl["G"] * -(-cx.nl // 64) for cx, S in zip(wl["complexes"], wl["n_swarms"])]
+    share = {}
+    for v, w in zip(chosen_names, blocks):
+        share[v] = share.get(v, 0) + w
+    variant_name = max(share, key=share.get)
+    kernel_name = "score32%s_kernel" % {"bcast": "b", "readlane": "r", "mfma": "m"}.get(
+        variant_name.split("/")[0].split("-")[0], "")
     out = {
         "metric": METRIC,
         "value": value,
@@ -294,10 +324,16 @@ def main():
                    "modes": list(wl["spec"]["modes"]), "guard_band_A2": 1e-4,
                    "parallelism": "1 GPU, all swarms of all complexes in one run"},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "score32_kernel", "launch_ms": score_ms,
+                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                     "traffic": traffic_bytes(args.workload),
+                     "kernel": kernel_name, "variant": variant_name, "launch_ms": score_ms,
                      "algorithmic": "8 FP32 flop per in-cutoff pair (0 < d2 <= 225), "
-                                    "%.4e pairs per launch" % pairs_per_launch},
+                                    "%.4e pairs per launch" % pairs_per_launch,
+                     "gather": {"achieved_GBps": gather_gbs, "probe_GBps": 4.0 * probe / 1e9,
+                                "frac": gather_gbs / (4.0 * probe / 1e9),
+                                "algorithmic": "4 bytes of table per in-cutoff pair, against "
+                                               "the measured random-gather rate over a "
+                                               "2.26 MB table"}},
         "cpu_baseline": cpu,
         "plain_fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
@@ -307,7 +343,8 @@ def main():
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
         "pairs_per_eval": pairs_per_launch / n_glow,
         "filter": filt,
-        "autotune": {"chosen": chosen, "ms_per_launch": tune_ms},
+        "autotune": {"chosen_per_complex": chosen_names,
+                     "ms_per_launch": ({k: sum(v) for k, v in tune_ms.items()} if tune_ms else None)},
     }
     print(json.dumps(out), flush=True)
 

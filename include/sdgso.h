@@ -140,17 +140,21 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
 /* Autotuner of the FP32 guard-band scoring kernel (no counterpart in the
  * reference, whose tile sizes are compile-time constants,
  * prepare_gso_task.cu:2503-2514).  sdgso_tune_variants() is the number of
- * compiled variants (gathers in flight per lane x register/occupancy
- * target); sdgso_run_tune scores the run's current poses with each variant
- * `repeats` times (luciferin and step counters untouched), keeps the fastest
- * for later steps, and returns the per-variant milliseconds per launch in
- * times_ms[sdgso_tune_variants()] (may be NULL) and the choice in *chosen. */
+ * compiled variants (filter form x gathers in flight per lane x
+ * register/occupancy target).  sdgso_run_tune scores the current poses of
+ * each complex of the run with each variant `repeats` times (luciferin and
+ * step counters untouched) and keeps, per complex, the fastest for later
+ * steps: times_ms[n_complex][sdgso_tune_variants()] receives the
+ * milliseconds per launch of each complex's share and chosen[n_complex] the
+ * choices (either may be NULL).  Complexes that share a variant share a
+ * launch.  sdgso_run_set_variant uses one variant for every complex (e.g. one
+ * chosen by an earlier sdgso_run_tune of the same workload) without timing;
+ * sdgso_run_variants reads the per-complex choices. */
 int sdgso_tune_variants(void);
 const char *sdgso_tune_variant_name(int variant);
 int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen);
-/* Use a given variant (e.g. one chosen by an earlier sdgso_run_tune of the
- * same workload) without timing. */
 int sdgso_run_set_variant(sdgso_run r, int variant);
+int sdgso_run_variants(sdgso_run r, int *variants);
 
 /* Filter statistics of the FP32 guard-band scoring kernel (no counterpart in
  * the reference, which scores every pair): scores the run's current poses
@@ -159,6 +163,11 @@ int sdgso_run_set_variant(sdgso_run r, int variant);
  * receptor tile, candidate tiles visited, receptor atoms passing the atom
  * test, pairs queued for exact evaluation, pairs inside the cutoff}. */
 int sdgso_run_filter_stats(sdgso_run r, int64_t *stats);
+
+/* Random-gather rate of the device over a float table of `table_floats`
+ * entries (no counterpart in the reference): the measured ceiling of the
+ * scoring kernels' table reads.  Returns gathers per second in *rate. */
+int sdgso_probe_gather(int64_t table_floats, double *rate);
 
 /* ---- host utilities (no device use) ---- */
 

@@ -192,7 +192,10 @@ struct sdgso_run_s {
     hipStream_t stream_h = nullptr;
     hipGraphExec_t graph = nullptr;
     bool profiling = false;
-    int variant = 0;  // FP32 scoring kernel variant (sdgso_run_tune)
+    // FP32 form: scoring-kernel variant of each complex (sdgso_run_tune) and
+    // the complex's range [work0[k], work0[k+1]) of the work list
+    std::vector<int> variants;
+    std::vector<size_t> work0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_score, ev_move;
     double t_score = 0, t_move = 0;
     int64_t n_score = 0, n_move = 0;
@@ -484,6 +487,21 @@ static ScoreArgs score_args(sdgso_run r) {
     return a;
 }
 
+// FP32 scoring of the work list: consecutive complexes that use the same
+// variant share a launch.
+static void launch_score32_run(sdgso_run r, const ScoreArgs &a) {
+    const int n = (int)r->variants.size();
+    for (int k0 = 0; k0 < n;) {
+        int k1 = k0 + 1;
+        while (k1 < n && r->variants[k1] == r->variants[k0]) k1++;
+        ScoreArgs part = a;
+        part.work = a.work + 2 * r->work0[k0];
+        part.n_work = (int)(r->work0[k1] - r->work0[k0]);
+        HIPCHK(launch_score32(part, r->variants[k0], r->stream_h));
+        k0 = k1;
+    }
+}
+
 // The scoring kernel (score64 or score32) sits between the two optional
 // events; finish32 (a per-glowworm epilogue) runs after them.
 static void enqueue_score(sdgso_run r, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
@@ -493,7 +511,7 @@ static void enqueue_score(sdgso_run r, hipEvent_t ev0 = nullptr, hipEvent_t ev1 
         HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
         if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
     } else {
-        HIPCHK(launch_score32(a, r->variant, r->stream_h));
+        launch_score32_run(r, a);
         if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
         HIPCHK(launch_finish32(a, r->stream_h));
     }
@@ -663,6 +681,7 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
                 fail(SDGSO_EINVAL, "run too large: " + std::to_string(total) +
                                        " (glowworm, ligand block) pairs");
             work.reserve(2 * total);
+            r->work0.assign(1, 0);
             size_t g0 = 0;
             for (int k = 0; k < n_complex; k++) {
                 const int nb = r->cxs[k]->dev.nlblocks;
@@ -673,9 +692,11 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
                         work.push_back(b);
                     }
                 g0 += ng;
+                r->work0.push_back(work.size() / 2);
             }
             r->work.upload(work);
             r->work.n = work.size() / 2;
+            r->variants.assign(n_complex, 0);
         }
         HIPCHK(hipStreamCreateWithFlags(&r->own_stream, hipStreamNonBlocking));
         r->stream_h = r->own_stream;
@@ -856,45 +877,51 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
         if (!r) fail(SDGSO_EINVAL, "null run");
         if (repeats <= 0) repeats = 1;
         ensure_device();
-        const int nv = score32_variants();
+        const int nv = score32_variants(), nc = r->n_complex;
         if (r->form != SDGSO_FP32_GUARD) {
             if (times_ms)
-                for (int v = 0; v < nv; v++) times_ms[v] = 0;
-            if (chosen) *chosen = r->variant;
+                for (int i = 0; i < nc * nv; i++) times_ms[i] = 0;
+            if (chosen)
+                for (int k = 0; k < nc; k++) chosen[k] = 0;
             return;
         }
         HIPCHK(hipStreamSynchronize(r->stream_h));
-        // score the current poses with every variant; luciferin and step are
-        // left alone (the next step rescores these poses anyway)
+        // score the current poses of each complex with every variant; luciferin
+        // and step are left alone (the next step rescores these poses anyway)
         ScoreArgs a = score_args(r);
         a.update_luciferin = 0;
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
-        double best = 1e300;
-        int best_v = r->variant;
-        for (int v = 0; v < nv; v++) {
-            HIPCHK(launch_score32(a, v, r->stream_h));  // warm-up
-            HIPCHK(hipEventRecord(e0, r->stream_h));
-            for (int k = 0; k < repeats; k++) HIPCHK(launch_score32(a, v, r->stream_h));
-            HIPCHK(hipEventRecord(e1, r->stream_h));
-            HIPCHK(hipEventSynchronize(e1));
-            float ms = 0;
-            HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-            const double t = ms / repeats;
-            if (times_ms) times_ms[v] = t;
-            if (t < best) {
-                best = t;
-                best_v = v;
+        for (int k = 0; k < nc; k++) {
+            ScoreArgs part = a;
+            part.work = a.work + 2 * r->work0[k];
+            part.n_work = (int)(r->work0[k + 1] - r->work0[k]);
+            double best = 1e300;
+            int best_v = 0;
+            for (int v = 0; v < nv; v++) {
+                HIPCHK(launch_score32(part, v, r->stream_h));  // warm-up
+                HIPCHK(hipEventRecord(e0, r->stream_h));
+                for (int i = 0; i < repeats; i++) HIPCHK(launch_score32(part, v, r->stream_h));
+                HIPCHK(hipEventRecord(e1, r->stream_h));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                const double t = ms / repeats;
+                if (times_ms) times_ms[(size_t)k * nv + v] = t;
+                if (t < best) {
+                    best = t;
+                    best_v = v;
+                }
             }
+            r->variants[k] = best_v;
+            if (chosen) chosen[k] = best_v;
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         HIPCHK(launch_finish32(a, r->stream_h));
         HIPCHK(hipStreamSynchronize(r->stream_h));
         drop_graph(r);
-        r->variant = best_v;
-        if (chosen) *chosen = best_v;
     });
 }
 
@@ -904,8 +931,16 @@ int sdgso_run_set_variant(sdgso_run r, int variant) {
         if (variant < 0 || variant >= score32_variants())
             fail(SDGSO_EINVAL, "variant out of [0, " + std::to_string(score32_variants()) + ")");
         HIPCHK(hipStreamSynchronize(r->stream_h));
-        if (variant != r->variant) drop_graph(r);
-        r->variant = variant;
+        drop_graph(r);
+        std::fill(r->variants.begin(), r->variants.end(), variant);
+    });
+}
+
+int sdgso_run_variants(sdgso_run r, int *variants) {
+    return guarded([&] {
+        if (!r || !variants) fail(SDGSO_EINVAL, "null argument");
+        for (int k = 0; k < r->n_complex; k++)
+            variants[k] = r->form == SDGSO_FP32_GUARD ? r->variants[k] : 0;
     });
 }
 
@@ -916,17 +951,53 @@ int sdgso_run_filter_stats(sdgso_run r, int64_t *stats) {
         ensure_device();
         HIPCHK(hipStreamSynchronize(r->stream_h));
         DevBuf<unsigned long long> st;
-        std::vector<unsigned long long> h(6, 0);
+        std::vector<unsigned long long> h(6 * kStatSlots, 0);
         st.upload(h);
         ScoreArgs a = score_args(r);
         a.update_luciferin = 0;
         a.guard_count = nullptr;
         a.stats = st.p;
         // the per-block partials are scratch: no epilogue, the run state stays as it was
-        HIPCHK(launch_score32(a, r->variant, r->stream_h));
+        launch_score32_run(r, a);
         HIPCHK(hipStreamSynchronize(r->stream_h));
-        HIPCHK(hipMemcpy(h.data(), st.p, 6 * sizeof(unsigned long long), hipMemcpyDeviceToHost));
-        for (int i = 0; i < 6; i++) stats[i] = (int64_t)h[i];
+        HIPCHK(hipMemcpy(h.data(), st.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+        for (int i = 0; i < 6; i++) {
+            unsigned long long sum = 0;
+            for (int k = 0; k < kStatSlots; k++) sum += h[(size_t)i * kStatSlots + k];
+            stats[i] = (int64_t)sum;
+        }
+    });
+}
+
+int sdgso_probe_gather(int64_t table_floats, double *rate) {
+    return guarded([&] {
+        if (table_floats <= 0 || table_floats > (1 << 30) || !rate)
+            fail(SDGSO_EINVAL, "bad probe arguments");
+        ensure_device();
+        DevBuf<float> table, out;
+        std::vector<float> ones((size_t)table_floats, 1.0f);
+        table.upload(ones);
+        int dev = 0, cus = 0;
+        HIPCHK(hipGetDevice(&dev));
+        HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        const int blocks = cus * 8, per_thread = 2048;
+        out.alloc((size_t)blocks * 256);
+        hipStream_t s = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(launch_gather_probe(table.p, (unsigned)table_floats, per_thread, out.p, blocks, s));
+        HIPCHK(hipEventRecord(e0, s));
+        HIPCHK(launch_gather_probe(table.p, (unsigned)table_floats, per_thread, out.p, blocks, s));
+        HIPCHK(hipEventRecord(e1, s));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+        *rate = (double)blocks * 256 * per_thread / (ms * 1e-3);
     });
 }
 
@@ -950,7 +1021,7 @@ int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int for
         if (form == SDGSO_FP64) {
             HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
         } else {
-            HIPCHK(launch_score32(a, r->variant, r->stream_h));
+            launch_score32_run(r, a);
             HIPCHK(launch_finish32(a, r->stream_h));
         }
         rc = sdgso_run_read(r, nullptr, nullptr, out_score, nullptr, nullptr, nullptr, nullptr,

@@ -268,12 +268,14 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
         a.partial_pairs[(size_t)x.gid * a.max_lblocks + x.b] = np;
         if (a.guard_count && ng) atomicAdd(a.guard_count, (unsigned long long)ng);
         if (a.stats) {
-            atomicAdd(a.stats + 0, 1ull);
-            atomicAdd(a.stats + 1, n_cand ? 1ull : 0ull);
-            atomicAdd(a.stats + 2, (unsigned long long)n_cand);
-            atomicAdd(a.stats + 3, (unsigned long long)n_near);
-            atomicAdd(a.stats + 4, (unsigned long long)n_queued);
-            atomicAdd(a.stats + 5, (unsigned long long)np);
+            // counters spread over kStatSlots slots each, so that waves rarely collide
+            unsigned long long *st = a.stats + (blockIdx.x % kStatSlots);
+            atomicAdd(st + 0 * kStatSlots, 1ull);
+            atomicAdd(st + 1 * kStatSlots, n_cand ? 1ull : 0ull);
+            atomicAdd(st + 2 * kStatSlots, (unsigned long long)n_cand);
+            atomicAdd(st + 3 * kStatSlots, (unsigned long long)n_near);
+            atomicAdd(st + 4 * kStatSlots, (unsigned long long)n_queued);
+            atomicAdd(st + 5 * kStatSlots, (unsigned long long)np);
         }
     }
 }

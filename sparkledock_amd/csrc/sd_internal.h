@@ -39,6 +39,7 @@ constexpr int kBlock = 64;             // ligand atoms per block = wavefront wid
 constexpr double kQuantScale = 1048576.0;  // 2^20: fixed-point units per A
 constexpr double kCutoff2 = 225.0;
 constexpr int kMaxTiles32 = 1024;      // FP32 form: receptor tiles per complex (65536 atoms)
+constexpr int kStatSlots = 256;        // slots per filter-statistics counter
 
 // Per-complex device descriptor (plain data; lives in a device array).
 struct DevComplex {
@@ -95,8 +96,9 @@ struct ScoreArgs {
     int max_lblocks;
     const int *work;        // FP32 form: work list of (glowworm, ligand block), or null
     int n_work;
-    // FP32 form, optional: filter statistics [waves, waves with candidate
-    // tiles, candidate tiles, near receptor atoms, queued pairs, in-cutoff pairs]
+    // FP32 form, optional: filter statistics [6][kStatSlots] (waves, waves
+    // with candidate tiles, candidate tiles, near receptor atoms, queued
+    // pairs, in-cutoff pairs), each the sum over its slots
     unsigned long long *stats;
 };
 
@@ -128,6 +130,8 @@ const char *score32_variant_name(int v);
 hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s);
 hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s);
 hipError_t launch_advance(int *d_k, hipStream_t s);
+hipError_t launch_gather_probe(const float *table, unsigned n, int per_thread, float *out,
+                               int blocks, hipStream_t s);
 
 void set_error(const std::string &msg);
 

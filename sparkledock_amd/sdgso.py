@@ -70,6 +70,8 @@ SIGNATURES = [
     ("sdgso_run_tune", ctypes.c_int, [_VP, ctypes.c_int, _D, _I]),
     ("sdgso_run_filter_stats", ctypes.c_int, [_VP, _I64]),
     ("sdgso_run_set_variant", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("sdgso_run_variants", ctypes.c_int, [_VP, _I]),
+    ("sdgso_probe_gather", ctypes.c_int, [ctypes.c_int64, _D]),
     ("sdgso_stream", ctypes.c_int,
      [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     ("sdgso_synth_molecule", ctypes.c_int,
@@ -251,14 +253,23 @@ class Run:
         return dict(score_ms=t[0], move_ms=t[1], score_launches=int(n[0]), move_launches=int(n[1]))
 
     def tune(self, repeats=1):
-        """Time every FP32 scoring-kernel variant on the current poses and keep
-        the fastest; returns (chosen, {variant name: ms per launch})."""
+        """Time every FP32 scoring-kernel variant on the current poses of each
+        complex and keep the fastest per complex; returns (chosen variant per
+        complex, {variant name: [ms per launch of each complex's share]})."""
         nv = lib().sdgso_tune_variants()
-        t = np.zeros(nv)
-        ch = ctypes.c_int(0)
-        _check(lib().sdgso_run_tune(self.h, int(repeats), _p(t), ctypes.byref(ch)))
-        names = [lib().sdgso_tune_variant_name(v).decode() for v in range(nv)]
-        return ch.value, dict(zip(names, t.tolist()))
+        nc = len(self.complexes)
+        t = np.zeros(nc * nv)
+        ch = np.zeros(nc, np.int32)
+        _check(lib().sdgso_run_tune(self.h, int(repeats), _p(t), _p(ch, _I)))
+        names = tune_variants()
+        t = t.reshape(nc, nv)
+        return ch.tolist(), {names[v]: t[:, v].tolist() for v in range(nv)}
+
+    def variants(self):
+        """The FP32 scoring-kernel variant of each complex."""
+        ch = np.zeros(len(self.complexes), np.int32)
+        _check(lib().sdgso_run_variants(self.h, _p(ch, _I)))
+        return ch.tolist()
 
     def set_variant(self, variant):
         _check(lib().sdgso_run_set_variant(self.h, int(variant)))
@@ -277,6 +288,13 @@ class Run:
         keys = ("waves", "waves_with_candidates", "candidate_tiles", "near_atoms", "queued_pairs",
                 "cutoff_pairs")
         return dict(zip(keys, (int(v) for v in s)))
+
+
+def probe_gather(table_floats=TABLE_LEN):
+    """Random-gather rate (gathers/s) of the device over a float table."""
+    r = ctypes.c_double(0)
+    _check(lib().sdgso_probe_gather(int(table_floats), ctypes.byref(r)))
+    return r.value
 
 
 def tune_variants():

@@ -348,3 +348,29 @@ def test_every_tile_a_candidate():
     run = sdgso.Run(pr.gpu, 1, len(poses), poses, None, 0, form=sdgso.FP32_GUARD)
     st = run.filter_stats()
     assert st["candidate_tiles"] == len(poses) * 313
+
+
+def test_tune_per_complex():
+    """The autotuner picks a variant per complex; the tuned batch gives the
+    FP64 form's pair counts and scores within the FP32 tolerance."""
+    ca = GC.case(19, 420, 200, 10, 10, 2, 24, swarm_radius=3.0)
+    cb = GC.case(20, 900, 90, 10, 10, 3, 24, swarm_radius=3.0)
+    pa, pb = GC.Pair(ca), GC.Pair(cb)
+    pos = [np.concatenate([ca["pos"].ravel(), cb["pos"].ravel()])]
+    u = np.concatenate([O.stream(0, 2, 2, 24).ravel(), O.stream(0, 3, 2, 24).ravel()])
+    runs = {}
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        runs[form] = sdgso.Run([pa.gpu, pb.gpu], [2, 3], 24, pos[0], u, 2, form=form)
+    chosen, times = runs[sdgso.FP32_GUARD].tune()
+    assert len(chosen) == 2 and chosen == runs[sdgso.FP32_GUARD].variants()
+    names = sdgso.tune_variants()
+    assert sorted(times) == sorted(names)
+    for k in range(2):
+        assert times[names[chosen[k]]][k] == min(t[k] for t in times.values())
+    for r in runs.values():
+        r.steps(1)
+    a, b = runs[sdgso.FP64].read(), runs[sdgso.FP32_GUARD].read()
+    assert np.array_equal(a["pairs"], b["pairs"])
+    assert np.allclose(a["score"], b["score"], rtol=0, atol=1e-4)
+    runs[sdgso.FP32_GUARD].set_variant(len(names) - 1)
+    assert runs[sdgso.FP32_GUARD].variants() == [len(names) - 1] * 2
