@@ -169,6 +169,74 @@ int sdgso_run_filter_stats(sdgso_run r, int64_t *stats);
  * scoring kernels' table reads.  Returns gathers per second in *rate. */
 int sdgso_probe_gather(int64_t table_floats, double *rate);
 
+/* ---- output side of a run ---- */
+
+/* Final coordinates of glowworms [first, first + count) of one complex of the
+ * run, from the current poses in FP64 in the reference's order (receptor
+ * modes; ligand modes, rotation, translation: prepare_gso_tasks.cc:882-920):
+ * rec_xyz[count][Nr][3], lig_xyz[count][Nl][3] (either may be NULL). */
+int sdgso_run_final_coords(sdgso_run r, int first, int count, double *rec_xyz, double *lig_xyz);
+
+/* Clustering of every swarm of complex `complex` of the run (BSAS,
+ * prepare_gso_tasks.cc:961-1080): glowworms in decreasing order of their last
+ * score (ties by index) join the first cluster whose representative (its
+ * first member) is within `cutoff` RMSD over the chosen backbone atoms of the
+ * receptor and the ligand (the reference uses CA and P atoms and 4 A), else
+ * found a new cluster.  cluster[S][G] receives each glowworm's cluster number
+ * within its swarm (in order of foundation) and representative[S][G] (may be
+ * NULL) the representative's glowworm index within the swarm. */
+int sdgso_run_cluster(sdgso_run r, int complex, const int *rec_backbone, int n_rec_backbone,
+                      const int *lig_backbone, int n_lig_backbone, double cutoff, int *cluster,
+                      int *representative);
+
+/* ---- molecular structures (host only) ---- */
+
+/* A structure read from a PDB file: the first model's ATOM/HETATM records,
+ * coordinates rounded through float as the reference stores them, residues
+ * split where (name, number, insertion) changes, chains where the chain id
+ * changes (read_input_structure.cc:131-307).  Flags drop OXT atoms, atoms
+ * whose name starts with H, and HOH residues (the setup's noxt / noh / now).
+ * An element outside C N O H S P CL MG FE PB SE F is an error (the
+ * reference exits). */
+typedef struct sdgso_structure_s *sdgso_structure;
+#define SDGSO_PDB_NO_OXT 1
+#define SDGSO_PDB_NO_H 2
+#define SDGSO_PDB_NO_WATER 4
+int sdgso_structure_read_pdb(const char *path, int flags, sdgso_structure *out);
+/* One CA atom of residue ALA i+1 in chain A per coordinate triple: the
+ * metadata of a synthetic Calpha chain, for writing and clustering. */
+int sdgso_structure_synthetic(int n_atoms, const double *xyz, sdgso_structure *out);
+int sdgso_structure_destroy(sdgso_structure s);
+int sdgso_structure_size(sdgso_structure s, int *n_atoms, int *n_residues, int *n_chains);
+int sdgso_structure_coords(sdgso_structure s, double *xyz);
+/* name[8], resname[8], chain[4] receive NUL-terminated strings (may be NULL). */
+int sdgso_structure_atom(sdgso_structure s, int atom, char *name, char *resname, int *resseq,
+                         char *chain);
+/* DFIRE atom types in [0, 168) (fastdfire.h tables as used by
+ * get_docking_model, prepare_gso_tasks.cc:31-69).  An atom outside the
+ * tables is an error (the reference reads an undefined type). */
+int sdgso_structure_dfire_types(sdgso_structure s, int *types);
+/* 1 for atoms of protein, modified-protein, DNA and RNA residues, else 0
+ * (get_nm_mask, complex.cc:149-209). */
+int sdgso_structure_nm_mask(sdgso_structure s, int *mask);
+/* Indices of the backbone atoms named CA or P, in file order (the atoms the
+ * clustering RMSD uses, prepare_gso_tasks.cc:1017-1050); *n receives the
+ * count (index may be NULL to count only). */
+int sdgso_structure_backbone(sdgso_structure s, int *index, int *n);
+/* The translation that centres the structure: minus the mean of the non-H
+ * atoms (init_center_translate, initialization.cc:3-31). */
+int sdgso_structure_centre(sdgso_structure s, double *translation);
+/* Atoms in the reference's write_pdb line format (prepare_gso_tasks.cc:
+ * 1279-1349), serial numbers from 1, coordinates from xyz (NULL: the
+ * structure's own); append != 0 appends to the file. */
+int sdgso_structure_write_pdb(sdgso_structure s, const double *xyz, const char *path, int append);
+
+/* Centre of the minimum-volume enclosing ellipsoid of the points
+ * (Khachiyan's iteration stopped when the weight update is at most
+ * `tolerance`; the reference uses 0.01), the ligand reference point of
+ * minimum_volume_ellipsoid, prepare_gso_tasks.cc:106-267. */
+int sdgso_reference_point(int n_atoms, const double *xyz, double tolerance, double *centre);
+
 /* ---- host utilities (no device use) ---- */
 
 /* The reference's random stream: u[s][k][g] = successive

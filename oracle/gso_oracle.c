@@ -458,3 +458,119 @@ void or_stream(uint32_t seed, int S, int n_steps, int G, double *u) {
         u[i] = r;
     }
 }
+
+/* ---------------------------------------------------------------- output side */
+
+double or_rmsd(const double *a, const double *b, int n) {
+    /* prepare_gso_tasks.cc:1258-1271 */
+    double r = 0.0;
+    for (int i = 0; i < n; i++) {
+        r += (a[3 * i] - b[3 * i]) * (a[3 * i] - b[3 * i]);
+        r += (a[3 * i + 1] - b[3 * i + 1]) * (a[3 * i + 1] - b[3 * i + 1]);
+        r += (a[3 * i + 2] - b[3 * i + 2]) * (a[3 * i + 2] - b[3 * i + 2]);
+    }
+    r /= n;
+    return sqrt(r);
+}
+
+void or_bsas(const double *coords, const double *score, int G, int n, double cutoff,
+             int *cluster, int *rep) {
+    /* prepare_gso_tasks.cc:975-1080 */
+    int *order = (int *)malloc(sizeof(int) * (size_t)G);
+    int *reps = (int *)malloc(sizeof(int) * (size_t)G);
+    for (int g = 0; g < G; g++) order[g] = g;
+    /* insertion sort: decreasing score, ties by index */
+    for (int i = 1; i < G; i++) {
+        const int v = order[i];
+        int j = i - 1;
+        while (j >= 0 && score[order[j]] < score[v]) {
+            order[j + 1] = order[j];
+            j--;
+        }
+        order[j + 1] = v;
+    }
+    int nc = 0;
+    for (int i = 0; i < G; i++) {
+        const int g = order[i];
+        int c = 0;
+        for (; c < nc; c++)
+            if (or_rmsd(coords + (size_t)reps[c] * n * 3, coords + (size_t)g * n * 3, n) <= cutoff)
+                break;
+        if (c == nc) reps[nc++] = g;
+        cluster[g] = c;
+        rep[g] = reps[c];
+    }
+    free(order);
+    free(reps);
+}
+
+/* 4x4 inverse by cofactors: inv = adj(A) / det(A) */
+static void inverse4_cofactor(const double *m, double *inv) {
+    double c[16];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) {
+            double s[9];
+            int k = 0;
+            for (int r = 0; r < 4; r++)
+                for (int q = 0; q < 4; q++)
+                    if (r != i && q != j) s[k++] = m[r * 4 + q];
+            const double d = s[0] * (s[4] * s[8] - s[5] * s[7]) - s[1] * (s[3] * s[8] - s[5] * s[6]) +
+                             s[2] * (s[3] * s[7] - s[4] * s[6]);
+            c[i * 4 + j] = ((i + j) % 2 ? -1.0 : 1.0) * d;
+        }
+    double det = 0.0;
+    for (int j = 0; j < 4; j++) det += m[j] * c[j];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) inv[i * 4 + j] = c[j * 4 + i] / det;
+}
+
+void or_mvee_centre(const double *xyz, int n, double tol, double *centre) {
+    /* prepare_gso_tasks.cc:106-267 (only the centre) */
+    double *u = (double *)malloc(sizeof(double) * (size_t)n);
+    double *nu = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) u[i] = 1.0 / n;
+    double err = 1.0 + tol;
+    while (err > tol) {
+        double V[16], A[16];
+        for (int a = 0; a < 4; a++)
+            for (int b = 0; b < 4; b++) {
+                double s = 0.0;
+                for (int i = 0; i < n; i++) {
+                    const double qa = a < 3 ? xyz[3 * i + a] : 1.0;
+                    const double qb = b < 3 ? xyz[3 * i + b] : 1.0;
+                    s += qa * (qb * u[i]);
+                }
+                V[a * 4 + b] = s;
+            }
+        inverse4_cofactor(V, A);
+        double best = 0.0;
+        int jb = 0;
+        for (int i = 0; i < n; i++) {
+            const double q[4] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2], 1.0};
+            double m = 0.0;
+            for (int k = 0; k < 4; k++) {
+                double t = 0.0;
+                for (int l = 0; l < 4; l++) t += A[l * 4 + k] * q[l];
+                m += q[k] * t;
+            }
+            if (m > best) {
+                best = m;
+                jb = i;
+            }
+        }
+        const double step = (best - 3.0 - 1.0) / ((3.0 + 1.0) * (best - 1.0));
+        double e2 = 0.0;
+        for (int i = 0; i < n; i++) nu[i] = (1.0 - step) * u[i];
+        nu[jb] += step;
+        for (int i = 0; i < n; i++) e2 += (nu[i] - u[i]) * (nu[i] - u[i]);
+        err = sqrt(e2);
+        for (int i = 0; i < n; i++) u[i] = nu[i];
+    }
+    for (int c = 0; c < 3; c++) {
+        double s = 0.0;
+        for (int i = 0; i < n; i++) s += xyz[3 * i + c] * u[i];
+        centre[c] = s;
+    }
+    free(u);
+    free(nu);
+}

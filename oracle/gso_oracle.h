@@ -20,6 +20,13 @@
  *     for its compiled 3 steps (10+10 modes) through oracle/_ref.
  *   or_stream: against the C++11 mt19937 known answer and libstdc++'s
  *     uniform_real_distribution output.
+ *   or_rmsd: bit-exact against the reference's cal_rmsd; closed forms
+ *     (identical sets, a rigid translation).
+ *   or_bsas: hand-built pose sets whose clusters follow from the distances
+ *     (the reference's loop is unreachable code, after its exit(0)).
+ *   or_mvee_centre: against the compiled reference's
+ *     minimum_volume_ellipsoid within 1e-9 A (its Eigen 4x4 inverse rounds
+ *     differently); translation equivariance; symmetric point sets.
  */
 #ifndef SD_GSO_ORACLE_H
 #define SD_GSO_ORACLE_H
@@ -129,6 +136,26 @@ void or_stream(uint32_t seed, int S, int n_steps, int G, double *u);
 
 /* Raw mt19937 outputs (for the known-answer test). */
 void or_mt19937(uint32_t seed, int n, uint32_t *out);
+
+/* Backbone RMSD of two coordinate sets of n atoms, summed per atom as
+ * dx^2, dy^2, dz^2 in order (cal_rmsd, prepare_gso_tasks.cc:1258-1271). */
+double or_rmsd(const double *a, const double *b, int n);
+
+/* Greedy clustering of G poses (prepare_gso_tasks.cc:975-1080): coords is
+ * [G][n][3] backbone coordinates, score [G]; poses in decreasing score order
+ * (ties by index) join the first cluster whose first member is within
+ * cutoff, else found a new one.  cluster[G] receives cluster numbers in order
+ * of foundation, rep[G] the representative of each pose's cluster.  Build
+ * decisions: every pose is counted once (the reference adds the top pose to
+ * its own cluster a second time); ties keep index order (the reference's
+ * std::sort leaves their order unspecified). */
+void or_bsas(const double *coords, const double *score, int G, int n, double cutoff,
+             int *cluster, int *rep);
+
+/* Centre of the minimum-volume enclosing ellipsoid (Khachiyan, stopped at a
+ * weight update <= tol; prepare_gso_tasks.cc:106-267), with a cofactor 4x4
+ * inverse. */
+void or_mvee_centre(const double *xyz, int n, double tol, double *centre);
 
 #ifdef __cplusplus
 }

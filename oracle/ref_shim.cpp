@@ -9,10 +9,23 @@
 #include <cstring>
 #include <vector>
 
+#include <fstream>
+#include <string>
+
 #include "complex.h"
 #include "fastdfire.h"
+#include "read_input_structure.h"
 #include "prepare_gso_tasks.h"
 #include "swarm_centers.h"
+
+// Defined in the reference sources without a header declaration.
+void minimum_volume_ellipsoid(double *center, double *poles, double *atom_coordinates,
+                              double precision, int num_atoms);
+void init_center_translate(double *center_coord, double *coord, vector<string> element,
+                           int *structure_size, const int num_atom, const int num_structure);
+void get_nm_mask(int *mask, vector<string> residue_names, map<string, string> STANDARD_TYPES,
+                 map<string, string> MODIFIED_TYPES, string *DNA_STANDARD_TYPES,
+                 string *RNA_STANDARD_TYPES, int *residue_atom_number, int *residue_size);
 
 template <class T>
 static bool rd(FILE *f, T *p, size_t n) {
@@ -58,6 +71,68 @@ int ref_select(double u, int *nei, double *prob, int nnei, int self, int G) {
     std::vector<int> sel(G, -1);
     select_random_neighbor(sel.data(), u, nei, prob, 0, self, nnei, 1);
     return sel[self];
+}
+
+// cal_rmsd (prepare_gso_tasks.cc:1258-1271)
+double ref_rmsd(const double *a, const double *b, int n) {
+    return cal_rmsd(const_cast<double *>(a), const_cast<double *>(b), n, n);
+}
+
+// minimum_volume_ellipsoid (prepare_gso_tasks.cc:106-267): the centre
+void ref_mvee(const double *xyz, int n, double tol, double *centre) {
+    double poles[18];
+    std::vector<double> c(xyz, xyz + 3 * (size_t)n);
+    minimum_volume_ellipsoid(centre, poles, c.data(), tol, n);
+}
+
+// ---- structures read by the reference's own reader (read_input_structure.cc) ----
+void *ref_pdb_open(const char *path, int noxt, int noh, int now) {
+    Complex *c = new Complex();
+    read_input_structure(*c, path, noxt != 0, noh != 0, now != 0, false, (int)std::strlen(path));
+    return c;
+}
+void ref_pdb_close(void *h) { delete static_cast<Complex *>(h); }
+int ref_pdb_natoms(void *h) { return static_cast<Complex *>(h)->num_atoms; }
+int ref_pdb_nresidues(void *h) { return static_cast<Complex *>(h)->num_res_type; }
+void ref_pdb_coords(void *h, double *xyz) {
+    Complex *c = static_cast<Complex *>(h);
+    std::memcpy(xyz, c->atom_coordinates, sizeof(double) * 3 * c->num_atoms);
+}
+// residue_atom_number[0..nres]: the first atom of every residue and the end
+void ref_pdb_residue_bounds(void *h, int *b) {
+    Complex *c = static_cast<Complex *>(h);
+    std::memcpy(b, c->residue_atom_number, sizeof(int) * (c->num_res_type + 1));
+}
+void ref_pdb_atom_name(void *h, int i, char *out) {
+    std::snprintf(out, 8, "%s", static_cast<Complex *>(h)->atom_name[i].c_str());
+}
+// get_docking_model (prepare_gso_tasks.cc:31-69): the first num_atoms types
+void ref_pdb_types(void *h, int *types) {
+    Complex *c = static_cast<Complex *>(h);
+    FastDifire fd;
+    std::vector<int> all((size_t)c->chain.size() * c->residue_atom_number[c->num_res_type] + 1, -1);
+    get_docking_model(all.data(), *c, fd);
+    std::memcpy(types, all.data(), sizeof(int) * c->num_atoms);
+}
+// get_nm_mask (complex.cc:149-209)
+void ref_pdb_mask(void *h, int *mask) {
+    Complex *c = static_cast<Complex *>(h);
+    get_nm_mask(c->mask, c->residue_name, c->STANDARD_TYPES, c->MODIFIED_TYPES,
+                c->DNA_STANDARD_TYPES, c->RNA_STANDARD_TYPES, c->residue_atom_number,
+                c->residue_size);
+    std::memcpy(mask, c->mask, sizeof(int) * c->num_atoms);
+}
+// init_center_translate (initialization.cc:3-31)
+void ref_pdb_centre(void *h, double *t) {
+    Complex *c = static_cast<Complex *>(h);
+    init_center_translate(t, c->atom_coordinates, c->elements, c->structure_size, c->num_atoms,
+                          c->num_structures);
+}
+// write_pdb (prepare_gso_tasks.cc:1279-1349) with the given coordinates
+void ref_pdb_write(void *h, const double *xyz, const char *path) {
+    Complex *c = static_cast<Complex *>(h);
+    std::ofstream f(path);
+    write_pdb(*c, f, const_cast<double *>(xyz));
 }
 
 // ---- whole-run harness over cal_gso_tasks_cpu (prepare_gso_tasks.cc:359-822) ----

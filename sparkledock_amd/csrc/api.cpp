@@ -1001,6 +1001,137 @@ int sdgso_probe_gather(int64_t table_floats, double *rate) {
     });
 }
 
+// ---------------------------------------------------------------- output side
+// The complex that owns glowworm `gid` of a run, and its first glowworm.
+static int complex_of(sdgso_run r, int gid, int *first) {
+    int g0 = 0;
+    for (int k = 0; k < r->n_complex; k++) {
+        const int ng = r->n_swarms[k] * r->G;
+        if (gid < g0 + ng) {
+            if (first) *first = g0;
+            return k;
+        }
+        g0 += ng;
+    }
+    fail(SDGSO_EINVAL, "glowworm index out of range");
+}
+
+int sdgso_run_final_coords(sdgso_run r, int first, int count, double *rec_xyz, double *lig_xyz) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (count <= 0) return;
+        if (first < 0 || first + count > r->N) fail(SDGSO_EINVAL, "glowworm range out of the run");
+        int g0 = 0;
+        const int k = complex_of(r, first, &g0);
+        if (complex_of(r, first + count - 1, nullptr) != k)
+            fail(SDGSO_EINVAL, "glowworm range spans two complexes");
+        ensure_device();
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        const int nr = r->cxs[k]->nr, nl = r->cxs[k]->nl, n = nr + nl;
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(count, (256u << 20) / (24 * (size_t)n)));
+        DevBuf<double> out;
+        out.alloc((size_t)chunk * n * 3);
+        std::vector<double> h((size_t)chunk * n * 3);
+        for (int c0 = 0; c0 < count; c0 += chunk) {
+            const int cn = std::min(chunk, count - c0);
+            CoordsArgs a{};
+            a.cx = r->dcx.p + k;
+            a.pos = r->pos.p;
+            a.P = r->P;
+            a.first = first + c0;
+            a.n_rec = nr;
+            a.n_lig = nl;
+            a.out = out.p;
+            HIPCHK(launch_coords(a, cn, r->stream_h));
+            HIPCHK(hipStreamSynchronize(r->stream_h));
+            HIPCHK(hipMemcpy(h.data(), out.p, (size_t)cn * n * 3 * sizeof(double), hipMemcpyDeviceToHost));
+            for (int g = 0; g < cn; g++) {
+                const double *src = h.data() + (size_t)g * n * 3;
+                if (rec_xyz)
+                    std::memcpy(rec_xyz + ((size_t)(c0 + g) * nr) * 3, src, sizeof(double) * 3 * nr);
+                if (lig_xyz)
+                    std::memcpy(lig_xyz + ((size_t)(c0 + g) * nl) * 3, src + 3 * (size_t)nr,
+                                sizeof(double) * 3 * nl);
+            }
+        }
+    });
+}
+
+int sdgso_run_cluster(sdgso_run r, int complex, const int *rec_backbone, int n_rec_backbone,
+                      const int *lig_backbone, int n_lig_backbone, double cutoff, int *cluster,
+                      int *representative) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (complex < 0 || complex >= r->n_complex) fail(SDGSO_EINVAL, "complex index out of range");
+        const sdgso_complex cx = r->cxs[complex];
+        const int nb = n_rec_backbone + n_lig_backbone;
+        if (n_rec_backbone < 0 || n_lig_backbone < 0 || nb <= 0)
+            fail(SDGSO_EINVAL, "no backbone atoms");
+        if ((n_rec_backbone && !rec_backbone) || (n_lig_backbone && !lig_backbone))
+            fail(SDGSO_EINVAL, "null backbone index array");
+        for (int i = 0; i < n_rec_backbone; i++)
+            if (rec_backbone[i] < 0 || rec_backbone[i] >= cx->nr) fail(SDGSO_EINVAL, "receptor backbone index out of range");
+        for (int i = 0; i < n_lig_backbone; i++)
+            if (lig_backbone[i] < 0 || lig_backbone[i] >= cx->nl) fail(SDGSO_EINVAL, "ligand backbone index out of range");
+        if (!cluster) fail(SDGSO_EINVAL, "null output");
+        ensure_device();
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        const int G = r->G, S = r->n_swarms[complex];
+        int g0 = 0;
+        for (int k = 0; k < complex; k++) g0 += r->n_swarms[k] * G;
+        std::vector<double> score((size_t)S * G);
+        HIPCHK(hipMemcpy(score.data(), r->score.p + g0, score.size() * sizeof(double), hipMemcpyDeviceToHost));
+        DevBuf<int> ridx, lidx;
+        if (n_rec_backbone) ridx.upload(rec_backbone, n_rec_backbone);
+        if (n_lig_backbone) lidx.upload(lig_backbone, n_lig_backbone);
+        // swarms per chunk: coordinates and RMSD matrices within a fixed budget
+        const size_t per_swarm = std::max((size_t)G * nb * 24, (size_t)G * G * 16);
+        const int chunk = (int)std::max<size_t>(1, std::min<size_t>(S, (512u << 20) / per_swarm));
+        DevBuf<double> bb, rm;
+        bb.alloc((size_t)chunk * G * nb * 3);
+        rm.alloc((size_t)chunk * G * G);
+        std::vector<double> h((size_t)chunk * G * G);
+        std::vector<int> order(G);
+        for (int s0 = 0; s0 < S; s0 += chunk) {
+            const int cs = std::min(chunk, S - s0);
+            CoordsArgs a{};
+            a.cx = r->dcx.p + complex;
+            a.pos = r->pos.p;
+            a.P = r->P;
+            a.first = g0 + s0 * G;
+            a.rec_idx = ridx.p;
+            a.n_rec = n_rec_backbone;
+            a.lig_idx = lidx.p;
+            a.n_lig = n_lig_backbone;
+            a.out = bb.p;
+            HIPCHK(launch_coords(a, cs * G, r->stream_h));
+            HIPCHK(launch_rmsd(bb.p, nb, G, cs, rm.p, r->stream_h));
+            HIPCHK(hipStreamSynchronize(r->stream_h));
+            HIPCHK(hipMemcpy(h.data(), rm.p, (size_t)cs * G * G * sizeof(double), hipMemcpyDeviceToHost));
+            for (int s = 0; s < cs; s++) {
+                const double *M = h.data() + (size_t)s * G * G;
+                const double *sc = score.data() + (size_t)(s0 + s) * G;
+                int *cl = cluster + (size_t)(s0 + s) * G;
+                int *rep = representative ? representative + (size_t)(s0 + s) * G : nullptr;
+                // decreasing score, ties by index (prepare_gso_tasks.cc:1273-1275, 987)
+                for (int g = 0; g < G; g++) order[g] = g;
+                std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return sc[x] > sc[y]; });
+                // greedy: join the first cluster whose representative is within the cutoff
+                // (prepare_gso_tasks.cc:1004-1080)
+                std::vector<int> reps;
+                for (int g : order) {
+                    int c = 0;
+                    for (; c < (int)reps.size(); c++)
+                        if (M[(size_t)reps[c] * G + g] <= cutoff) break;
+                    if (c == (int)reps.size()) reps.push_back(g);
+                    cl[g] = c;
+                    if (rep) rep[g] = reps[c];
+                }
+            }
+        }
+    });
+}
+
 // ---------------------------------------------------------------- scoring plugin
 int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int form,
                       double guard_band, double *out_score, double *out_ref, int64_t *out_pairs) {

@@ -122,6 +122,19 @@ struct SelectMoveArgs {
     int do_select, do_move;
 };
 
+// Final coordinates of chosen atoms for a range of glowworms of one complex.
+struct CoordsArgs {
+    const DevComplex *cx;   // the complex (device)
+    const double *pos;      // run poses [N][P]
+    int P;
+    int first;              // first glowworm (global index)
+    const int *rec_idx;     // [n_rec] receptor atoms, or null for 0..n_rec-1
+    int n_rec;
+    const int *lig_idx;     // [n_lig] ligand atoms, or null for 0..n_lig-1
+    int n_lig;
+    double *out;            // [count][n_rec + n_lig][3]
+};
+
 // Launchers (defined in the .hip files).
 hipError_t launch_score64(const ScoreArgs &a, int max_nl, hipStream_t s);
 hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s);
@@ -130,6 +143,8 @@ const char *score32_variant_name(int v);
 hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s);
 hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s);
 hipError_t launch_advance(int *d_k, hipStream_t s);
+hipError_t launch_coords(const CoordsArgs &a, int count, hipStream_t s);
+hipError_t launch_rmsd(const double *bb, int nb, int G, int n_swarms, double *rmsd, hipStream_t s);
 hipError_t launch_gather_probe(const float *table, unsigned n, int per_thread, float *out,
                                int blocks, hipStream_t s);
 

@@ -12,15 +12,25 @@
 //   - the run always executes on the GPU through libsdgso (USE_CUDA is not
 //     consulted); "form": "fp64" selects the plain FP64 kernels, otherwise the
 //     FP32 guard-band form runs;
-//   - inputs are synthetic complexes ("synthetic_receptor_atoms",
-//     "synthetic_ligand_atoms", "synthetic_seed"); PDB input is not read by
-//     this build;
+//   - inputs are either the reference's "receptor_pdb" / "ligand_pdb" (read
+//     with its noxt / noh / now filters, DFIRE-typed, masked, centred, the
+//     ligand reference point from the minimum-volume ellipsoid) or synthetic
+//     complexes ("synthetic_receptor_atoms", "synthetic_ligand_atoms",
+//     "synthetic_seed"); the DFIRE table is read from "dfire_table" (one
+//     value per line, the DCparams layout, stored as FP32) or drawn from the
+//     synthetic N(0,1)*0.5 table; starting poses are this build's seeded
+//     placement around the receptor (the reference's surface sampling is not
+//     reproduced);
 //   - every swarm_<i>/gso_<STEPS>.out is written fresh (the reference's mmap
-//     writer appends to stale files, prepare_gso_task.cu:2860-2904).
+//     writer appends to stale files, prepare_gso_task.cu:2860-2904);
+//   - swarm_<i>/cluster.repr is written after the run (the reference's
+//     clustering code, prepare_gso_tasks.cc:961-1120, sits after its exit(0));
+//     "write_pdbs": true also writes swarm_<i>/lightdock_<g>.pdb.
 // Output format of each line follows prepare_gso_task.cu:2868-2888.
 #include <getopt.h>
 #include <sys/stat.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -113,6 +123,68 @@ void check(int rc, const char *what) {
 // std::to_string(double) formatting ("%f"), as the reference writer uses.
 std::string f6(double v) { return std::to_string(v); }
 
+// swarm_<s>/cluster.repr: "cluster:size:score:representative:lightdock_<rep>.pdb"
+// per cluster in order of foundation (prepare_gso_tasks.cc:1084-1110; the
+// score printed is the representative's own, and every member is counted once).
+void write_clusters(int s, int G, const int *cl, const int *repr, const std::vector<double> &score) {
+    const std::string path = "swarm_" + std::to_string(s) + "/cluster.repr";
+    std::vector<int> size(G, 0), first(G, -1);
+    int nc = 0;
+    for (int g = 0; g < G; g++) {
+        size[cl[g]]++;
+        first[cl[g]] = repr[g];
+        nc = std::max(nc, cl[g] + 1);
+    }
+    std::string buf;
+    char line[160];
+    for (int c = 0; c < nc; c++) {
+        const int id = first[c];
+        std::snprintf(line, sizeof line, "%d:%d:%g:%d:lightdock_%d.pdb\n", c, size[c],
+                      score[(size_t)s * G + id], id, id);
+        buf += line;
+    }
+    FILE *f = std::fopen(path.c_str(), "wb");
+    if (!f) {
+        std::perror(path.c_str());
+        std::exit(1);
+    }
+    std::fwrite(buf.data(), 1, buf.size(), f);
+    std::fclose(f);
+}
+
+std::string resolve(const std::string &p, const std::string &setup_path) {
+    struct stat sb;
+    if (stat(p.c_str(), &sb) == 0 || p.empty() || p[0] == '/') return p;
+    const size_t slash = setup_path.rfind('/');
+    if (slash == std::string::npos) return p;
+    return setup_path.substr(0, slash + 1) + p;
+}
+
+// Reads a structure and the arrays the run needs from it.
+struct Molecule {
+    sdgso_structure s = nullptr;
+    int n = 0;
+    std::vector<double> xyz;
+    std::vector<int> type, mask;
+    double translation[3] = {0, 0, 0};
+};
+
+Molecule read_molecule(const std::string &path, int flags) {
+    Molecule m;
+    check(sdgso_structure_read_pdb(path.c_str(), flags, &m.s), path.c_str());
+    check(sdgso_structure_size(m.s, &m.n, nullptr, nullptr), "structure size");
+    m.xyz.resize(3 * (size_t)m.n);
+    m.type.resize(m.n);
+    m.mask.resize(m.n);
+    check(sdgso_structure_coords(m.s, m.xyz.data()), "coordinates");
+    check(sdgso_structure_dfire_types(m.s, m.type.data()), "DFIRE typing");
+    check(sdgso_structure_nm_mask(m.s, m.mask.data()), "normal-mode mask");
+    check(sdgso_structure_centre(m.s, m.translation), "centring");
+    for (int i = 0; i < m.n; i++)
+        for (int k = 0; k < 3; k++) m.xyz[3 * (size_t)i + k] += m.translation[k];
+    return m;
+}
+
 void write_swarm(int s, int steps, int G, int P, const std::vector<double> &pos,
                  const std::vector<double> &lucif, const std::vector<int> &nnei,
                  const std::vector<double> &vis, const std::vector<double> &score) {
@@ -162,19 +234,12 @@ int main(int argc, char **argv) {
         std::fprintf(stderr, "%s\n", e.what());
         return 1;
     }
-    const int nr = (int)st.num("synthetic_receptor_atoms", 0);
-    const int nl = (int)st.num("synthetic_ligand_atoms", 0);
-    if (nr <= 0 || nl <= 0) {
-        std::fprintf(stderr,
-                     "this build docks synthetic complexes: set synthetic_receptor_atoms and "
-                     "synthetic_ligand_atoms in %s (PDB input is not read)\n",
-                     setup_path.c_str());
-        return 1;
-    }
+    const std::string rec_pdb = st.text("receptor_pdb", ""), lig_pdb = st.text("ligand_pdb", "");
+    const bool from_pdb = !rec_pdb.empty() && !lig_pdb.empty() && rec_pdb != "null" && lig_pdb != "null";
     const uint64_t seed = (uint64_t)st.num("synthetic_seed", 0);
     const bool use_anm = st.num("use_anm", 0) != 0;
-    const int nmr = use_anm ? (int)st.num("anm_rec", 10) : 0;
-    const int nml = use_anm ? (int)st.num("anm_lig", 10) : 0;
+    int nmr = use_anm ? (int)st.num("anm_rec", 10) : 0;
+    int nml = use_anm ? (int)st.num("anm_lig", 10) : 0;
     const int S = (int)st.num("swarms", 1);
     const int G = (int)st.num("glowworms", 200);
     const double swarm_radius = st.num("swarm_radius", 10.0);
@@ -192,30 +257,99 @@ int main(int argc, char **argv) {
     prm.step_rotation = st.num("rotation_step", prm.step_rotation);
     prm.step_nmodes = st.num("nmodes_step", prm.step_nmodes);
 
-    std::vector<double> rx(3 * (size_t)nr), lx(3 * (size_t)nl), rm((size_t)nmr * nr * 3 + 1),
-        lm((size_t)nml * nl * 3 + 1);
-    std::vector<int> rt(nr), lt(nl);
-    std::vector<float> table(SDGSO_TABLE_LEN);
-    check(sdgso_synth_molecule(1000 * seed + 1, nr, nmr, rx.data(), rt.data(), rm.data()), "synth");
-    check(sdgso_synth_molecule(1000 * seed + 2, nl, nml, lx.data(), lt.data(), lm.data()), "synth");
-    check(sdgso_synth_table(7, table.data()), "synth table");
+    int nr = 0, nl = 0;
+    std::vector<double> rx, lx, rm(1), lm(1);
+    std::vector<int> rt, lt, rmask, lmask;
+    sdgso_structure rs = nullptr, ls = nullptr;
     double ref[3] = {0, 0, 0};
-    for (int i = 0; i < nl; i++)
-        for (int k = 0; k < 3; k++) ref[k] += lx[3 * (size_t)i + k] / nl;
+    if (from_pdb) {
+        if (use_anm && (nmr > 0 || nml > 0)) {
+            std::fprintf(stderr, "normal modes of PDB inputs are not computed by this build: set "
+                                 "use_anm to false in %s\n", setup_path.c_str());
+            return 1;
+        }
+        const int flags = (st.num("noxt", 0) != 0 ? SDGSO_PDB_NO_OXT : 0) |
+                          (st.num("noh", 0) != 0 ? SDGSO_PDB_NO_H : 0) |
+                          (st.num("now", 0) != 0 ? SDGSO_PDB_NO_WATER : 0);
+        Molecule rmol = read_molecule(resolve(rec_pdb, setup_path), flags);
+        Molecule lmol = read_molecule(resolve(lig_pdb, setup_path), flags);
+        nr = rmol.n;
+        nl = lmol.n;
+        rx = rmol.xyz;
+        lx = lmol.xyz;
+        rt = rmol.type;
+        lt = lmol.type;
+        rmask = rmol.mask;
+        lmask = lmol.mask;
+        rs = rmol.s;
+        ls = lmol.s;
+        check(sdgso_reference_point(nl, lx.data(), 0.01, ref), "reference point");
+    } else {
+        nr = (int)st.num("synthetic_receptor_atoms", 0);
+        nl = (int)st.num("synthetic_ligand_atoms", 0);
+        if (nr <= 0 || nl <= 0) {
+            std::fprintf(stderr,
+                         "no input complex: set receptor_pdb and ligand_pdb, or "
+                         "synthetic_receptor_atoms and synthetic_ligand_atoms, in %s\n",
+                         setup_path.c_str());
+            return 1;
+        }
+        rx.resize(3 * (size_t)nr);
+        lx.resize(3 * (size_t)nl);
+        rm.resize((size_t)nmr * nr * 3 + 1);
+        lm.resize((size_t)nml * nl * 3 + 1);
+        rt.resize(nr);
+        lt.resize(nl);
+        check(sdgso_synth_molecule(1000 * seed + 1, nr, nmr, rx.data(), rt.data(), rm.data()), "synth");
+        check(sdgso_synth_molecule(1000 * seed + 2, nl, nml, lx.data(), lt.data(), lm.data()), "synth");
+        for (int i = 0; i < nl; i++)
+            for (int k = 0; k < 3; k++) ref[k] += lx[3 * (size_t)i + k] / nl;
+        check(sdgso_structure_synthetic(nr, rx.data(), &rs), "structure");
+        check(sdgso_structure_synthetic(nl, lx.data(), &ls), "structure");
+    }
+    std::vector<float> table(SDGSO_TABLE_LEN, 0.0f);
+    const std::string table_path = st.text("dfire_table", "");
+    if (!table_path.empty() && table_path != "null") {
+        std::ifstream tin(resolve(table_path, setup_path));
+        if (!tin) {
+            std::fprintf(stderr, "cannot open the DFIRE table %s\n", table_path.c_str());
+            return 1;
+        }
+        double v;
+        int n = 0;
+        while (n < SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS && (tin >> v)) table[n++] = (float)v;
+        if (n != SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS) {
+            std::fprintf(stderr, "the DFIRE table %s holds %d values, expected %d\n",
+                         table_path.c_str(), n, SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS);
+            return 1;
+        }
+    } else {
+        check(sdgso_synth_table(7, table.data()), "synth table");
+    }
     const int P = 7 + nmr + nml;
     std::vector<double> pos((size_t)S * G * P);
-    const double centre = std::cbrt(nr * 11.4) + 0.5 * std::cbrt(nl * 11.4);
+    // swarm centres on a sphere around the receptor, at its radius plus half the ligand's
+    double rrad = 0, lrad = 0;
+    for (int i = 0; i < nr; i++)
+        rrad = std::max(rrad, std::sqrt(rx[3 * i] * rx[3 * i] + rx[3 * i + 1] * rx[3 * i + 1] +
+                                        rx[3 * i + 2] * rx[3 * i + 2]));
+    for (int i = 0; i < nl; i++)
+        lrad = std::max(lrad, std::sqrt(lx[3 * i] * lx[3 * i] + lx[3 * i + 1] * lx[3 * i + 1] +
+                                        lx[3 * i + 2] * lx[3 * i + 2]));
+    const double centre = from_pdb ? rrad + 0.5 * lrad
+                                   : std::cbrt(nr * 11.4) + 0.5 * std::cbrt(nl * 11.4);
     check(sdgso_synth_poses((uint64_t)st.num("starting_points_seed", 1000 * seed + 3), S, G, nmr,
                             nml, centre, swarm_radius, pos.data()),
-          "synth poses");
+          "starting poses");
     // the reference's stream: mt19937(0), [swarm][step][glowworm]
     std::vector<double> u((size_t)S * steps * G + 1);
     check(sdgso_stream(0, S, steps, G, u.data()), "stream");
 
     sdgso_complex cx = nullptr;
-    check(sdgso_complex_create(nr, rx.data(), rt.data(), nullptr, nmr, nmr ? rm.data() : nullptr, nl,
-                               lx.data(), lt.data(), nullptr, nml, nml ? lm.data() : nullptr, ref,
-                               table.data(), &cx),
+    check(sdgso_complex_create(nr, rx.data(), rt.data(), rmask.empty() ? nullptr : rmask.data(), nmr,
+                               nmr ? rm.data() : nullptr, nl, lx.data(), lt.data(),
+                               lmask.empty() ? nullptr : lmask.data(), nml,
+                               nml ? lm.data() : nullptr, ref, table.data(), &cx),
           "sdgso_complex_create");
     sdgso_run run = nullptr;
     check(sdgso_run_create(1, &cx, &S, G, pos.data(), u.data(), steps, &prm, form, 1e-4, &run),
@@ -233,8 +367,31 @@ int main(int argc, char **argv) {
           "sdgso_run_read");
     std::printf("saving pos file\n");
     for (int s = 0; s < S; s++) write_swarm(s, steps, G, P, pos, lucif, nnei, vis, score);
+    // clustering over the CA / P atoms of both molecules, 4 A (prepare_gso_tasks.cc:961-1080)
+    int nrb = 0, nlb = 0;
+    check(sdgso_structure_backbone(rs, nullptr, &nrb), "backbone");
+    check(sdgso_structure_backbone(ls, nullptr, &nlb), "backbone");
+    std::vector<int> rb(nrb + 1), lb(nlb + 1), cl(N), repr(N);
+    check(sdgso_structure_backbone(rs, rb.data(), &nrb), "backbone");
+    check(sdgso_structure_backbone(ls, lb.data(), &nlb), "backbone");
+    check(sdgso_run_cluster(run, 0, rb.data(), nrb, lb.data(), nlb, 4.0, cl.data(), repr.data()),
+          "sdgso_run_cluster");
+    for (int s = 0; s < S; s++) write_clusters(s, G, cl.data() + (size_t)s * G, repr.data() + (size_t)s * G, score);
+    if (st.num("write_pdbs", 0) != 0) {
+        std::vector<double> rc((size_t)G * nr * 3), lc((size_t)G * nl * 3);
+        for (int s = 0; s < S; s++) {
+            check(sdgso_run_final_coords(run, s * G, G, rc.data(), lc.data()), "final coordinates");
+            for (int g = 0; g < G; g++) {
+                const std::string path = "swarm_" + std::to_string(s) + "/lightdock_" + std::to_string(g) + ".pdb";
+                check(sdgso_structure_write_pdb(rs, rc.data() + (size_t)g * nr * 3, path.c_str(), 0), "write_pdb");
+                check(sdgso_structure_write_pdb(ls, lc.data() + (size_t)g * nl * 3, path.c_str(), 1), "write_pdb");
+            }
+        }
+    }
     std::printf("simulation execution:%f\n", secs);
     sdgso_run_destroy(run);
     sdgso_complex_destroy(cx);
+    sdgso_structure_destroy(rs);
+    sdgso_structure_destroy(ls);
     return 0;
 }

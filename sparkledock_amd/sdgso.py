@@ -72,6 +72,22 @@ SIGNATURES = [
     ("sdgso_run_set_variant", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_run_variants", ctypes.c_int, [_VP, _I]),
     ("sdgso_probe_gather", ctypes.c_int, [ctypes.c_int64, _D]),
+    ("sdgso_run_final_coords", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _D, _D]),
+    ("sdgso_run_cluster", ctypes.c_int,
+     [_VP, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double, _I, _I]),
+    ("sdgso_structure_read_pdb", ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.POINTER(_VP)]),
+    ("sdgso_structure_synthetic", ctypes.c_int, [ctypes.c_int, _D, ctypes.POINTER(_VP)]),
+    ("sdgso_structure_destroy", ctypes.c_int, [_VP]),
+    ("sdgso_structure_size", ctypes.c_int, [_VP, _I, _I, _I]),
+    ("sdgso_structure_coords", ctypes.c_int, [_VP, _D]),
+    ("sdgso_structure_atom", ctypes.c_int,
+     [_VP, ctypes.c_int, ctypes.c_char_p, ctypes.c_char_p, _I, ctypes.c_char_p]),
+    ("sdgso_structure_dfire_types", ctypes.c_int, [_VP, _I]),
+    ("sdgso_structure_nm_mask", ctypes.c_int, [_VP, _I]),
+    ("sdgso_structure_backbone", ctypes.c_int, [_VP, _I, _I]),
+    ("sdgso_structure_centre", ctypes.c_int, [_VP, _D]),
+    ("sdgso_structure_write_pdb", ctypes.c_int, [_VP, _D, ctypes.c_char_p, ctypes.c_int]),
+    ("sdgso_reference_point", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_double, _D]),
     ("sdgso_stream", ctypes.c_int,
      [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     ("sdgso_synth_molecule", ctypes.c_int,
@@ -279,6 +295,32 @@ class Run:
         _check(lib().sdgso_run_guard_stats(self.h, _p(s, _I64)))
         return dict(recomputed=int(s[0]))
 
+    def final_coords(self, first, count):
+        """Final FP64 coordinates of glowworms [first, first + count) (one
+        complex): (receptor [count][Nr][3], ligand [count][Nl][3])."""
+        k = 0
+        g0 = 0
+        while first >= g0 + self.n_swarms[k] * self.G:
+            g0 += self.n_swarms[k] * self.G
+            k += 1
+        cx = self.complexes[k]
+        rec = np.zeros((count, cx.nr, 3))
+        lig = np.zeros((count, cx.nl, 3))
+        _check(lib().sdgso_run_final_coords(self.h, int(first), int(count), _p(rec), _p(lig)))
+        return rec, lig
+
+    def cluster(self, complex_index, rec_backbone, lig_backbone, cutoff=4.0):
+        """Clusters of every swarm of one complex: (cluster [S][G],
+        representative [S][G]), both within the swarm."""
+        rb = np.ascontiguousarray(rec_backbone, dtype=np.int32)
+        lb = np.ascontiguousarray(lig_backbone, dtype=np.int32)
+        S = self.n_swarms[complex_index]
+        cl = np.zeros((S, self.G), np.int32)
+        rep = np.zeros((S, self.G), np.int32)
+        _check(lib().sdgso_run_cluster(self.h, int(complex_index), _p(rb, _I), len(rb), _p(lb, _I),
+                                       len(lb), float(cutoff), _p(cl, _I), _p(rep, _I)))
+        return cl, rep
+
     def filter_stats(self):
         """Counts of one FP32 scoring pass over the current poses: waves
         (glowworm x ligand block), waves with candidate tiles, candidate tiles,
@@ -288,6 +330,94 @@ class Run:
         keys = ("waves", "waves_with_candidates", "candidate_tiles", "near_atoms", "queued_pairs",
                 "cutoff_pairs")
         return dict(zip(keys, (int(v) for v in s)))
+
+
+PDB_NO_OXT, PDB_NO_H, PDB_NO_WATER = 1, 2, 4
+
+
+class Structure:
+    """A molecular structure (atoms with PDB metadata), host only."""
+
+    def __init__(self, handle):
+        self.h = handle
+        n = ctypes.c_int(0)
+        r = ctypes.c_int(0)
+        c = ctypes.c_int(0)
+        _check(lib().sdgso_structure_size(self.h, ctypes.byref(n), ctypes.byref(r), ctypes.byref(c)))
+        self.n_atoms, self.n_residues, self.n_chains = n.value, r.value, c.value
+
+    @classmethod
+    def read_pdb(cls, path, noxt=True, noh=True, now=True):
+        h = _VP()
+        flags = (PDB_NO_OXT if noxt else 0) | (PDB_NO_H if noh else 0) | (PDB_NO_WATER if now else 0)
+        _check(lib().sdgso_structure_read_pdb(str(path).encode(), flags, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def synthetic(cls, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        h = _VP()
+        _check(lib().sdgso_structure_synthetic(len(xyz), _p(xyz), ctypes.byref(h)))
+        return cls(h)
+
+    def coords(self):
+        x = np.zeros((self.n_atoms, 3))
+        _check(lib().sdgso_structure_coords(self.h, _p(x)))
+        return x
+
+    def atom(self, i):
+        name = ctypes.create_string_buffer(8)
+        res = ctypes.create_string_buffer(8)
+        chain = ctypes.create_string_buffer(4)
+        num = ctypes.c_int(0)
+        _check(lib().sdgso_structure_atom(self.h, int(i), name, res, ctypes.byref(num), chain))
+        return name.value.decode(), res.value.decode(), num.value, chain.value.decode()
+
+    def dfire_types(self):
+        t = np.zeros(self.n_atoms, np.int32)
+        _check(lib().sdgso_structure_dfire_types(self.h, _p(t, _I)))
+        return t
+
+    def nm_mask(self):
+        m = np.zeros(self.n_atoms, np.int32)
+        _check(lib().sdgso_structure_nm_mask(self.h, _p(m, _I)))
+        return m
+
+    def backbone(self):
+        n = ctypes.c_int(0)
+        _check(lib().sdgso_structure_backbone(self.h, None, ctypes.byref(n)))
+        idx = np.zeros(n.value, np.int32)
+        _check(lib().sdgso_structure_backbone(self.h, _p(idx, _I), ctypes.byref(n)))
+        return idx
+
+    def centre(self):
+        t = np.zeros(3)
+        _check(lib().sdgso_structure_centre(self.h, _p(t)))
+        return t
+
+    def write_pdb(self, path, xyz=None, append=False):
+        x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64)
+        _check(lib().sdgso_structure_write_pdb(self.h, _p(x), str(path).encode(), 1 if append else 0))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().sdgso_structure_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def reference_point(xyz, tolerance=0.01):
+    """Centre of the minimum-volume enclosing ellipsoid (the ligand
+    reference point)."""
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    c = np.zeros(3)
+    _check(lib().sdgso_reference_point(len(xyz), _p(xyz), float(tolerance), _p(c)))
+    return c
 
 
 def probe_gather(table_floats=TABLE_LEN):

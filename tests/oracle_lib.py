@@ -83,6 +83,10 @@ def lib():
         L.or_select.restype = ctypes.c_int
         L.or_select.argtypes = [ctypes.c_double, _I, _D, ctypes.c_int, ctypes.c_int]
         L.or_default_params.argtypes = [ctypes.POINTER(OrParams)]
+        L.or_rmsd.restype = ctypes.c_double
+        L.or_rmsd.argtypes = [_D, _D, ctypes.c_int]
+        L.or_bsas.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, _I, _I]
+        L.or_mvee_centre.argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
         L.or_state_alloc.restype = ctypes.c_int
         L.or_state_alloc.argtypes = [ctypes.POINTER(OrState), ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.c_int]
@@ -252,8 +256,99 @@ def ref():
         R.ref_probabilities.argtypes = [_D, _I, _D, ctypes.c_int, ctypes.c_double]
         R.ref_select.restype = ctypes.c_int
         R.ref_select.argtypes = [ctypes.c_double, _I, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        R.ref_rmsd.restype = ctypes.c_double
+        R.ref_rmsd.argtypes = [_D, _D, ctypes.c_int]
+        R.ref_mvee.argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]
+        R.ref_pdb_open.restype = ctypes.c_void_p
+        R.ref_pdb_open.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        R.ref_pdb_close.argtypes = [ctypes.c_void_p]
+        R.ref_pdb_natoms.restype = ctypes.c_int
+        R.ref_pdb_natoms.argtypes = [ctypes.c_void_p]
+        R.ref_pdb_nresidues.restype = ctypes.c_int
+        R.ref_pdb_nresidues.argtypes = [ctypes.c_void_p]
+        for name, t in (("ref_pdb_coords", _D), ("ref_pdb_residue_bounds", _I),
+                        ("ref_pdb_types", _I), ("ref_pdb_mask", _I), ("ref_pdb_centre", _D)):
+            getattr(R, name).argtypes = [ctypes.c_void_p, t]
+        R.ref_pdb_atom_name.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_char_p]
+        R.ref_pdb_write.argtypes = [ctypes.c_void_p, _D, ctypes.c_char_p]
         _ref = R
     return _ref
+
+
+def rmsd(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return lib().or_rmsd(_p(a), _p(b), len(a))
+
+
+def bsas(coords, score, cutoff=4.0):
+    """Oracle clustering of one swarm: coords [G][n][3], score [G] ->
+    (cluster [G], representative [G])."""
+    coords = np.ascontiguousarray(coords, dtype=np.float64)
+    score = np.ascontiguousarray(score, dtype=np.float64)
+    G, n = coords.shape[:2]
+    cl = np.zeros(G, np.int32)
+    rep = np.zeros(G, np.int32)
+    lib().or_bsas(_p(coords), _p(score), G, n, float(cutoff), _p(cl, _I), _p(rep, _I))
+    return cl, rep
+
+
+def mvee_centre(xyz, tol=0.01):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    c = np.zeros(3)
+    lib().or_mvee_centre(_p(xyz), len(xyz), float(tol), _p(c))
+    return c
+
+
+class RefStructure:
+    """A structure read by the reference's own reader (oracle/_ref)."""
+
+    def __init__(self, path, noxt=True, noh=True, now=True):
+        self.h = ref().ref_pdb_open(path.encode(), int(noxt), int(noh), int(now))
+        self.n = ref().ref_pdb_natoms(self.h)
+        self.nres = ref().ref_pdb_nresidues(self.h)
+
+    def coords(self):
+        x = np.zeros((self.n, 3))
+        ref().ref_pdb_coords(self.h, _p(x))
+        return x
+
+    def residue_bounds(self):
+        b = np.zeros(self.nres + 1, np.int32)
+        ref().ref_pdb_residue_bounds(self.h, _p(b, _I))
+        return b
+
+    def names(self):
+        buf = ctypes.create_string_buffer(8)
+        out = []
+        for i in range(self.n):
+            ref().ref_pdb_atom_name(self.h, i, buf)
+            out.append(buf.value.decode())
+        return out
+
+    def types(self):
+        t = np.zeros(self.n, np.int32)
+        ref().ref_pdb_types(self.h, _p(t, _I))
+        return t
+
+    def mask(self):
+        m = np.zeros(self.n, np.int32)
+        ref().ref_pdb_mask(self.h, _p(m, _I))
+        return m
+
+    def centre(self):
+        c = np.zeros(3)
+        ref().ref_pdb_centre(self.h, _p(c))
+        return c
+
+    def write(self, xyz, path):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        ref().ref_pdb_write(self.h, _p(xyz), path.encode())
+
+    def close(self):
+        if self.h:
+            ref().ref_pdb_close(self.h)
+            self.h = None
 
 
 def ref_cpu_run(cx, pos, S, G, workdir):

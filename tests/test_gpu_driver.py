@@ -2,7 +2,9 @@
 reference CLI of source/simulation/run_simulation.cc): it writes one
 swarm_<i>/gso_<STEPS>.out per swarm in the reference line format
 (prepare_gso_task.cu:2868-2888) and the values equal the same run through the
-C ABI."""
+C ABI; it clusters every swarm into swarm_<i>/cluster.repr
+(prepare_gso_tasks.cc:1084-1110) and optionally writes the final poses as
+PDB files; it reads PDB inputs (the 2X9A benchmark complex, tests/golden/pdb)."""
 import json
 import math
 import os
@@ -17,6 +19,16 @@ pytestmark = pytest.mark.gpu
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 DRIVER = os.path.join(ROOT, "sparkledock_amd", "lightdock")
+PDB_DIR = os.path.join(ROOT, "tests", "golden", "pdb")
+
+
+def read_clusters(path):
+    out = []
+    for line in open(path):
+        c, size, score, rep, name = line.rstrip("\n").split(":")
+        assert name == "lightdock_%s.pdb" % rep
+        out.append((int(c), int(size), float(score), int(rep)))
+    return out
 
 
 @pytest.mark.parametrize("form", ["fp64", "fp32_guard"])
@@ -63,6 +75,15 @@ def test_lightdock_writes_swarm_files(tmp_path, form):
             assert nnei == got["nnei"][i]
             assert abs(vis - got["vision"][i]) < 1.5e-6
             assert abs(score - got["score"][i]) < 1.5e-6
+    # clusters: every glowworm once, representatives and sizes from the same clustering
+    cl, rep = run.cluster(0, np.arange(nr, dtype=np.int32), np.arange(nl, dtype=np.int32), 4.0)
+    for s in range(S):
+        rows = read_clusters(tmp_path / ("swarm_%d" % s) / "cluster.repr")
+        assert [r[0] for r in rows] == list(range(len(rows)))
+        assert sum(r[1] for r in rows) == G
+        for c, size, score, r in rows:
+            assert size == int(np.sum(cl[s] == c)) and rep[s][cl[s] == c][0] == r
+            assert abs(score - got["score"][s * G + r]) <= 1e-5 * max(1.0, abs(score))
 
 
 def test_lightdock_usage_errors(tmp_path):
@@ -73,3 +94,48 @@ def test_lightdock_usage_errors(tmp_path):
     r = subprocess.run([DRIVER, "-f", "setup.json", "-s", "1"], cwd=tmp_path,
                        stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=60)
     assert r.returncode == 1 and "synthetic" in r.stderr
+
+
+def test_lightdock_pdb_inputs(tmp_path):
+    """The 2X9A complex from its PDB files: typing, mask, centring and the
+    reference point feed the run; results equal the same run through the C
+    ABI; the final poses are written as PDB files."""
+    S, G, steps = 2, 12, 3
+    setup = {"swarms": S, "glowworms": G, "use_anm": False, "swarm_radius": 5.0,
+             "receptor_pdb": os.path.join(PDB_DIR, "2X9A_r_u.pdb"),
+             "ligand_pdb": os.path.join(PDB_DIR, "2X9A_l_u.pdb"),
+             "noh": True, "noxt": True, "now": True, "starting_points_seed": 11,
+             "form": "fp32_guard", "write_pdbs": True}
+    (tmp_path / "setup.json").write_text(json.dumps(setup))
+    r = subprocess.run([DRIVER, "-f", "setup.json", "-s", str(steps)], cwd=tmp_path,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    res = ranking.read_run(str(tmp_path), steps)
+    assert sorted(res) == list(range(S))
+    # the same inputs through the C ABI
+    rs = sdgso.Structure.read_pdb(setup["receptor_pdb"])
+    ls = sdgso.Structure.read_pdb(setup["ligand_pdb"])
+    rx = rs.coords() + rs.centre()
+    lx = ls.coords() + ls.centre()
+    ref = sdgso.reference_point(lx)
+    table = sdgso.synth_table(7)
+    cx = sdgso.Complex(rx, rs.dfire_types(), lx, ls.dfire_types(), table, ref, None, None,
+                       rs.nm_mask(), ls.nm_mask())
+    # the driver's radii: sqrt(x*x + y*y + z*z) per atom, in that order
+    rrad = np.max(np.sqrt(rx[:, 0] * rx[:, 0] + rx[:, 1] * rx[:, 1] + rx[:, 2] * rx[:, 2]))
+    lrad = np.max(np.sqrt(lx[:, 0] * lx[:, 0] + lx[:, 1] * lx[:, 1] + lx[:, 2] * lx[:, 2]))
+    pos = sdgso.synth_poses(11, S, G, 0, 0, rrad + 0.5 * lrad, 5.0)
+    run = sdgso.Run(cx, S, G, pos, sdgso.stream(0, S, steps, G), steps, form=sdgso.FP32_GUARD)
+    run.steps(steps, graph=True)
+    got = run.read()
+    for s in range(S):
+        for g, rec in enumerate(res[s]):
+            assert np.allclose(rec[0], got["pos"][s * G + g], atol=1.5e-6)
+            assert abs(rec[6] - got["score"][s * G + g]) < 1.5e-6
+        rows = read_clusters(tmp_path / ("swarm_%d" % s) / "cluster.repr")
+        assert sum(r[1] for r in rows) == G
+    # the final poses as PDB files: receptor then ligand, at the run's final coordinates
+    rc, lc = run.final_coords(G + 3, 1)
+    back = sdgso.Structure.read_pdb(tmp_path / "swarm_1" / "lightdock_3.pdb")
+    assert back.n_atoms == rs.n_atoms + ls.n_atoms
+    assert np.max(np.abs(back.coords() - np.concatenate([rc[0], lc[0]]))) <= 5.1e-4
