@@ -231,6 +231,21 @@ int sdgso_structure_centre(sdgso_structure s, double *translation);
  * structure's own); append != 0 appends to the file. */
 int sdgso_structure_write_pdb(sdgso_structure s, const double *xyz, const char *path, int append);
 
+/* Normal modes by the anisotropic network model (cal_anm.cc:61-219): xyz
+ * [N][3] (in/out) are the structure's centred coordinates, re-centred in
+ * place on the mean of all atoms as the reference does before the modes;
+ * the network nodes are the CA atoms (else C4), one per residue, with a 15 A
+ * cutoff; the lowest n_modes non-zero modes (eigenvalue >= 1e-6) are given
+ * to every atom of their residue, normalised over the 3N components and
+ * scaled by sqrt(N) * rmsd / coef * sqrt(1 / eigenvalue), coef the mean norm
+ * of n_draws random extents (libstdc++ default_random_engine seeded 324324,
+ * N(0, 1)) weighted by 1 / eigenvalue.  modes[n_modes][N][3]; eigenvalues
+ * [n_modes] (may be NULL).  Each mode's sign is fixed so that its largest
+ * component is positive (the reference takes the eigensolver's).  Uses the
+ * GPU (rocSOLVER). */
+int sdgso_structure_anm(sdgso_structure s, double *xyz, int n_modes, double rmsd, int n_draws,
+                        double *modes, double *eigenvalues);
+
 /* Centre of the minimum-volume enclosing ellipsoid of the points
  * (Khachiyan's iteration stopped when the weight update is at most
  * `tolerance`; the reference uses 0.01), the ligand reference point of

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include <fstream>
+#include <random>
 #include <string>
 
 #include "complex.h"
@@ -71,6 +72,16 @@ int ref_select(double u, int *nei, double *prob, int nnei, int self, int G) {
     std::vector<int> sel(G, -1);
     select_random_neighbor(sel.data(), u, nei, prob, 0, self, nnei, 1);
     return sel[self];
+}
+
+// libstdc++'s std::default_random_engine + std::normal_distribution<double>,
+// the generator cal_anm.cc:166-184 draws its extents from (the third-party
+// arithmetic the ANM restatement follows).
+void ref_std_normal(unsigned seed, int n, double mu, double sigma, double *out) {
+    std::default_random_engine g;
+    g.seed(seed);
+    std::normal_distribution<double> d(mu, sigma);
+    for (int i = 0; i < n; i++) out[i] = d(g);
 }
 
 // cal_rmsd (prepare_gso_tasks.cc:1258-1271)

@@ -13,8 +13,9 @@
 //     consulted); "form": "fp64" selects the plain FP64 kernels, otherwise the
 //     FP32 guard-band form runs;
 //   - inputs are either the reference's "receptor_pdb" / "ligand_pdb" (read
-//     with its noxt / noh / now filters, DFIRE-typed, masked, centred, the
-//     ligand reference point from the minimum-volume ellipsoid) or synthetic
+//     with its noxt / noh / now filters, DFIRE-typed, masked, centred, normal
+//     modes from the ANM when use_anm is set, the ligand reference point from
+//     the minimum-volume ellipsoid) or synthetic
 //     complexes ("synthetic_receptor_atoms", "synthetic_ligand_atoms",
 //     "synthetic_seed"); the DFIRE table is read from "dfire_table" (one
 //     value per line, the DCparams layout, stored as FP32) or drawn from the
@@ -263,11 +264,6 @@ int main(int argc, char **argv) {
     sdgso_structure rs = nullptr, ls = nullptr;
     double ref[3] = {0, 0, 0};
     if (from_pdb) {
-        if (use_anm && (nmr > 0 || nml > 0)) {
-            std::fprintf(stderr, "normal modes of PDB inputs are not computed by this build: set "
-                                 "use_anm to false in %s\n", setup_path.c_str());
-            return 1;
-        }
         const int flags = (st.num("noxt", 0) != 0 ? SDGSO_PDB_NO_OXT : 0) |
                           (st.num("noh", 0) != 0 ? SDGSO_PDB_NO_H : 0) |
                           (st.num("now", 0) != 0 ? SDGSO_PDB_NO_WATER : 0);
@@ -283,6 +279,20 @@ int main(int argc, char **argv) {
         lmask = lmol.mask;
         rs = rmol.s;
         ls = lmol.s;
+        // normal modes (cal_anm.cc:61-219, run_simulation.cc:79-92); the coordinates
+        // of a molecule with modes are re-centred on all its atoms
+        if (nmr > 0) {
+            rm.assign((size_t)nmr * nr * 3, 0.0);
+            check(sdgso_structure_anm(rs, rx.data(), nmr, st.num("anm_rec_rmsd", 0.5), S * G,
+                                      rm.data(), nullptr),
+                  "receptor normal modes");
+        }
+        if (nml > 0) {
+            lm.assign((size_t)nml * nl * 3, 0.0);
+            check(sdgso_structure_anm(ls, lx.data(), nml, st.num("anm_lig_rmsd", 0.5), S * G,
+                                      lm.data(), nullptr),
+                  "ligand normal modes");
+        }
         check(sdgso_reference_point(nl, lx.data(), 0.01, ref), "reference point");
     } else {
         nr = (int)st.num("synthetic_receptor_atoms", 0);

@@ -88,6 +88,8 @@ SIGNATURES = [
     ("sdgso_structure_centre", ctypes.c_int, [_VP, _D]),
     ("sdgso_structure_write_pdb", ctypes.c_int, [_VP, _D, ctypes.c_char_p, ctypes.c_int]),
     ("sdgso_reference_point", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_double, _D]),
+    ("sdgso_structure_anm", ctypes.c_int,
+     [_VP, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D]),
     ("sdgso_stream", ctypes.c_int,
      [ctypes.c_uint32, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
     ("sdgso_synth_molecule", ctypes.c_int,
@@ -394,6 +396,16 @@ class Structure:
         t = np.zeros(3)
         _check(lib().sdgso_structure_centre(self.h, _p(t)))
         return t
+
+    def anm(self, xyz, n_modes, rmsd=0.5, n_draws=1):
+        """Normal modes (needs the GPU): returns (re-centred coordinates [N][3],
+        modes [n_modes][N][3], eigenvalues [n_modes])."""
+        x = np.array(xyz, dtype=np.float64, order="C", copy=True)
+        modes = np.zeros((n_modes, self.n_atoms, 3))
+        ev = np.zeros(n_modes)
+        _check(lib().sdgso_structure_anm(self.h, _p(x), int(n_modes), float(rmsd), int(n_draws),
+                                         _p(modes), _p(ev)))
+        return x, modes, ev
 
     def write_pdb(self, path, xyz=None, append=False):
         x = None if xyz is None else np.ascontiguousarray(xyz, dtype=np.float64)

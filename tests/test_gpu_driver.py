@@ -96,12 +96,15 @@ def test_lightdock_usage_errors(tmp_path):
     assert r.returncode == 1 and "synthetic" in r.stderr
 
 
-def test_lightdock_pdb_inputs(tmp_path):
-    """The 2X9A complex from its PDB files: typing, mask, centring and the
-    reference point feed the run; results equal the same run through the C
-    ABI; the final poses are written as PDB files."""
+@pytest.mark.parametrize("use_anm", [False, True])
+def test_lightdock_pdb_inputs(tmp_path, use_anm):
+    """The 2X9A complex from its PDB files: typing, mask, centring, normal
+    modes and the reference point feed the run; results equal the same run
+    through the C ABI; the final poses are written as PDB files."""
     S, G, steps = 2, 12, 3
-    setup = {"swarms": S, "glowworms": G, "use_anm": False, "swarm_radius": 5.0,
+    nm = 10 if use_anm else 0
+    setup = {"swarms": S, "glowworms": G, "use_anm": use_anm, "anm_rec": 10, "anm_lig": 10,
+             "anm_rec_rmsd": 0.5, "anm_lig_rmsd": 0.5, "swarm_radius": 5.0,
              "receptor_pdb": os.path.join(PDB_DIR, "2X9A_r_u.pdb"),
              "ligand_pdb": os.path.join(PDB_DIR, "2X9A_l_u.pdb"),
              "noh": True, "noxt": True, "now": True, "starting_points_seed": 11,
@@ -117,14 +120,18 @@ def test_lightdock_pdb_inputs(tmp_path):
     ls = sdgso.Structure.read_pdb(setup["ligand_pdb"])
     rx = rs.coords() + rs.centre()
     lx = ls.coords() + ls.centre()
+    rm = lm = None
+    if use_anm:
+        rx, rm, _ = rs.anm(rx, nm, 0.5, S * G)
+        lx, lm, _ = ls.anm(lx, nm, 0.5, S * G)
     ref = sdgso.reference_point(lx)
     table = sdgso.synth_table(7)
-    cx = sdgso.Complex(rx, rs.dfire_types(), lx, ls.dfire_types(), table, ref, None, None,
+    cx = sdgso.Complex(rx, rs.dfire_types(), lx, ls.dfire_types(), table, ref, rm, lm,
                        rs.nm_mask(), ls.nm_mask())
     # the driver's radii: sqrt(x*x + y*y + z*z) per atom, in that order
     rrad = np.max(np.sqrt(rx[:, 0] * rx[:, 0] + rx[:, 1] * rx[:, 1] + rx[:, 2] * rx[:, 2]))
     lrad = np.max(np.sqrt(lx[:, 0] * lx[:, 0] + lx[:, 1] * lx[:, 1] + lx[:, 2] * lx[:, 2]))
-    pos = sdgso.synth_poses(11, S, G, 0, 0, rrad + 0.5 * lrad, 5.0)
+    pos = sdgso.synth_poses(11, S, G, nm, nm, rrad + 0.5 * lrad, 5.0)
     run = sdgso.Run(cx, S, G, pos, sdgso.stream(0, S, steps, G), steps, form=sdgso.FP32_GUARD)
     run.steps(steps, graph=True)
     got = run.read()

@@ -211,3 +211,46 @@ def test_bsas_hand_cases():
     # equal scores keep index order
     cl, rep = O.bsas(chain, np.zeros(5))
     assert list(rep) == [0, 0, 2, 2, 4]
+
+
+# ---------------------------------------------------------------- normal-mode oracle
+
+def _anm_oracle():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location(
+        "anm_oracle", os.path.join(os.path.dirname(O.__file__), "..", "oracle", "anm_oracle.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_minstd_known_answer():
+    A = _anm_oracle()
+    g = A.MinStdRand0(1)
+    for _ in range(9999):
+        g()
+    assert g() == 1043618065  # C++11 [rand.predef]: 10000th output of minstd_rand0
+
+
+@need_ref
+def test_normal_distribution_matches_libstdcxx():
+    A = _anm_oracle()
+    want = np.zeros(501)
+    O.ref().ref_std_normal(324324, len(want), 0.0, 1.0, O._p(want))
+    g = A.MinStdRand0(324324)
+    nd = A.NormalDistribution(0.0, 1.0)
+    assert np.array_equal(np.array([nd(g) for _ in range(len(want))]), want)
+
+
+def test_anm_hessian_properties():
+    A = _anm_oracle()
+    rng = np.random.default_rng(8)
+    c = rng.normal(size=(30, 3)) * 3.0  # dense enough that the network is rigid
+    H = A.hessian(c)
+    assert np.array_equal(H, H.T)
+    assert np.allclose(H.sum(axis=1), 0.0, atol=1e-12)
+    w = np.linalg.eigvalsh(H)
+    assert np.sum(np.abs(w) < 1e-8) == 6  # rigid-body modes of a connected network
+    # a rigid translation is a zero mode exactly
+    t = np.tile([0.3, -1.2, 0.7], 30)
+    assert np.allclose(H @ t, 0.0, atol=1e-12)
