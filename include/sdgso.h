@@ -246,6 +246,32 @@ int sdgso_structure_write_pdb(sdgso_structure s, const double *xyz, const char *
 int sdgso_structure_anm(sdgso_structure s, double *xyz, int n_modes, double rmsd, int n_draws,
                         double *modes, double *eigenvalues);
 
+/* k-means++ seeding from std::mt19937(seed) (kmeans_init_centers3,
+ * kmeans.cc:126-187) followed by `iterations` Lloyd steps (run_kmeans3,
+ * kmeans.cc:191-243) over points[n][3]: centres[k][3], group[n] (may be
+ * NULL). */
+int sdgso_kmeans(const double *points, int n, int k, uint32_t seed, int iterations, double *centres,
+                 int *group);
+
+/* Starting poses of `swarms` swarms of `glowworms` glowworms around the
+ * receptor (calculate_surface_point and calculate_start_pos,
+ * cal_anm.cc:541-900): rec_xyz and lig_xyz are the structures' coordinates
+ * before centring, rec_translation the receptor's centring translation.
+ * Swarm centres come from k-means over the atoms of surface residues, 100
+ * sphere points around each centroid at a distance set by the ligand's
+ * diameter, without points near another centroid or within 3 A of the
+ * receptor, and a second k-means; each glowworm gets a translation uniform in
+ * the swarm_radius sphere and a uniform rotation (std::mt19937
+ * start_points_seed) and N(0, 1) mode extents (std::default_random_engine
+ * anm_seed).  centres[swarms][3] (may be NULL); pos[swarms][glowworms][7 +
+ * n_modes_rec + n_modes_lig].  The number of swarms must be given (the
+ * reference's count from the receptor's SASA needs FreeSASA). */
+int sdgso_starting_poses(sdgso_structure receptor, const double *rec_xyz, int n_lig,
+                         const double *lig_xyz, const double *rec_translation, int swarms,
+                         int glowworms, double swarm_radius, uint32_t start_points_seed,
+                         uint32_t anm_seed, int n_modes_rec, int n_modes_lig, double *centres,
+                         double *pos);
+
 /* Centre of the minimum-volume enclosing ellipsoid of the points
  * (Khachiyan's iteration stopped when the weight update is at most
  * `tolerance`; the reference uses 0.01), the ligand reference point of

@@ -15,6 +15,7 @@
 
 #include "complex.h"
 #include "fastdfire.h"
+#include "kmeans.h"
 #include "read_input_structure.h"
 #include "prepare_gso_tasks.h"
 #include "swarm_centers.h"
@@ -82,6 +83,13 @@ void ref_std_normal(unsigned seed, int n, double mu, double sigma, double *out) 
     g.seed(seed);
     std::normal_distribution<double> d(mu, sigma);
     for (int i = 0; i < n; i++) out[i] = d(g);
+}
+
+// kmeans_init_centers3 + run_kmeans3 (kmeans.cc:126-243)
+void ref_kmeans(const double *points, int n, int k, int seed, int iters, double *centres, int *group) {
+    std::vector<double> p(points, points + 3 * (size_t)n);
+    kmeans_init_centers3(centres, p.data(), n, k, seed);
+    run_kmeans3(group, p.data(), centres, iters, n, k);
 }
 
 // cal_rmsd (prepare_gso_tasks.cc:1258-1271)

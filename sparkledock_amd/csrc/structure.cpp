@@ -327,6 +327,7 @@ static void pdb_line(std::string &out, const AtomRec &a, int serial, double x, d
 // Access for the normal-mode code (anm.cpp).
 int structure_atoms(sdgso_structure s) { return (int)s->atoms.size(); }
 const char *structure_atom_name(sdgso_structure s, int i) { return s->atoms[i].name.c_str(); }
+const char *structure_residue_name(sdgso_structure s, int r) { return s->residues[r].name.c_str(); }
 void structure_residues(sdgso_structure s, std::vector<int> &first, std::vector<int> &count) {
     first.clear();
     count.clear();

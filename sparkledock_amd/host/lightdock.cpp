@@ -15,13 +15,13 @@
 //   - inputs are either the reference's "receptor_pdb" / "ligand_pdb" (read
 //     with its noxt / noh / now filters, DFIRE-typed, masked, centred, normal
 //     modes from the ANM when use_anm is set, the ligand reference point from
-//     the minimum-volume ellipsoid) or synthetic
-//     complexes ("synthetic_receptor_atoms", "synthetic_ligand_atoms",
-//     "synthetic_seed"); the DFIRE table is read from "dfire_table" (one
-//     value per line, the DCparams layout, stored as FP32) or drawn from the
-//     synthetic N(0,1)*0.5 table; starting poses are this build's seeded
-//     placement around the receptor (the reference's surface sampling is not
-//     reproduced);
+//     the minimum-volume ellipsoid, starting poses from the reference's
+//     surface sampling with "swarms" given, as FreeSASA is not available to
+//     derive it) or synthetic complexes ("synthetic_receptor_atoms",
+//     "synthetic_ligand_atoms", "synthetic_seed", seeded placement around
+//     the receptor); the DFIRE table is read from "dfire_table" (one value
+//     per line, the DCparams layout, stored as FP32) or drawn from the
+//     synthetic N(0,1)*0.5 table;
 //   - every swarm_<i>/gso_<STEPS>.out is written fresh (the reference's mmap
 //     writer appends to stale files, prepare_gso_task.cu:2860-2904);
 //   - swarm_<i>/cluster.repr is written after the run (the reference's
@@ -165,7 +165,7 @@ std::string resolve(const std::string &p, const std::string &setup_path) {
 struct Molecule {
     sdgso_structure s = nullptr;
     int n = 0;
-    std::vector<double> xyz;
+    std::vector<double> raw, xyz;  // as read; centred
     std::vector<int> type, mask;
     double translation[3] = {0, 0, 0};
 };
@@ -181,6 +181,7 @@ Molecule read_molecule(const std::string &path, int flags) {
     check(sdgso_structure_dfire_types(m.s, m.type.data()), "DFIRE typing");
     check(sdgso_structure_nm_mask(m.s, m.mask.data()), "normal-mode mask");
     check(sdgso_structure_centre(m.s, m.translation), "centring");
+    m.raw = m.xyz;
     for (int i = 0; i < m.n; i++)
         for (int k = 0; k < 3; k++) m.xyz[3 * (size_t)i + k] += m.translation[k];
     return m;
@@ -259,7 +260,7 @@ int main(int argc, char **argv) {
     prm.step_nmodes = st.num("nmodes_step", prm.step_nmodes);
 
     int nr = 0, nl = 0;
-    std::vector<double> rx, lx, rm(1), lm(1);
+    std::vector<double> rx, lx, rm(1), lm(1), pos;
     std::vector<int> rt, lt, rmask, lmask;
     sdgso_structure rs = nullptr, ls = nullptr;
     double ref[3] = {0, 0, 0};
@@ -279,6 +280,13 @@ int main(int argc, char **argv) {
         lmask = lmol.mask;
         rs = rmol.s;
         ls = lmol.s;
+        // starting poses from the coordinates as read (run_simulation.cc:59-71)
+        pos.resize((size_t)S * G * (7 + nmr + nml));
+        check(sdgso_starting_poses(rs, rmol.raw.data(), nl, lmol.raw.data(), rmol.translation, S, G,
+                                   swarm_radius, (uint32_t)st.num("starting_points_seed", 324324),
+                                   (uint32_t)st.num("anm_seed", 324324), nmr, nml, nullptr,
+                                   pos.data()),
+              "starting poses");
         // normal modes (cal_anm.cc:61-219, run_simulation.cc:79-92); the coordinates
         // of a molecule with modes are re-centred on all its atoms
         if (nmr > 0) {
@@ -337,20 +345,14 @@ int main(int argc, char **argv) {
         check(sdgso_synth_table(7, table.data()), "synth table");
     }
     const int P = 7 + nmr + nml;
-    std::vector<double> pos((size_t)S * G * P);
-    // swarm centres on a sphere around the receptor, at its radius plus half the ligand's
-    double rrad = 0, lrad = 0;
-    for (int i = 0; i < nr; i++)
-        rrad = std::max(rrad, std::sqrt(rx[3 * i] * rx[3 * i] + rx[3 * i + 1] * rx[3 * i + 1] +
-                                        rx[3 * i + 2] * rx[3 * i + 2]));
-    for (int i = 0; i < nl; i++)
-        lrad = std::max(lrad, std::sqrt(lx[3 * i] * lx[3 * i] + lx[3 * i + 1] * lx[3 * i + 1] +
-                                        lx[3 * i + 2] * lx[3 * i + 2]));
-    const double centre = from_pdb ? rrad + 0.5 * lrad
-                                   : std::cbrt(nr * 11.4) + 0.5 * std::cbrt(nl * 11.4);
-    check(sdgso_synth_poses((uint64_t)st.num("starting_points_seed", 1000 * seed + 3), S, G, nmr,
-                            nml, centre, swarm_radius, pos.data()),
-          "starting poses");
+    if (!from_pdb) {
+        // synthetic complexes: swarm centres on a sphere around the globule
+        pos.resize((size_t)S * G * P);
+        const double centre = std::cbrt(nr * 11.4) + 0.5 * std::cbrt(nl * 11.4);
+        check(sdgso_synth_poses((uint64_t)st.num("starting_points_seed", 1000 * seed + 3), S, G, nmr,
+                                nml, centre, swarm_radius, pos.data()),
+              "starting poses");
+    }
     // the reference's stream: mt19937(0), [swarm][step][glowworm]
     std::vector<double> u((size_t)S * steps * G + 1);
     check(sdgso_stream(0, S, steps, G, u.data()), "stream");

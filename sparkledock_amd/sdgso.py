@@ -88,6 +88,11 @@ SIGNATURES = [
     ("sdgso_structure_centre", ctypes.c_int, [_VP, _D]),
     ("sdgso_structure_write_pdb", ctypes.c_int, [_VP, _D, ctypes.c_char_p, ctypes.c_int]),
     ("sdgso_reference_point", ctypes.c_int, [ctypes.c_int, _D, ctypes.c_double, _D]),
+    ("sdgso_kmeans", ctypes.c_int,
+     [_D, ctypes.c_int, ctypes.c_int, ctypes.c_uint32, ctypes.c_int, _D, _I]),
+    ("sdgso_starting_poses", ctypes.c_int,
+     [_VP, _D, ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_uint32,
+      ctypes.c_uint32, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("sdgso_structure_anm", ctypes.c_int,
      [_VP, _D, ctypes.c_int, ctypes.c_double, ctypes.c_int, _D, _D]),
     ("sdgso_stream", ctypes.c_int,
@@ -421,6 +426,32 @@ class Structure:
             self.close()
         except Exception:
             pass
+
+
+def kmeans(points, k, seed=324324, iterations=100):
+    """k-means++ seeding and Lloyd iterations: (centres [k][3], group [n])."""
+    p = np.ascontiguousarray(points, dtype=np.float64)
+    c = np.zeros((k, 3))
+    g = np.zeros(len(p), np.int32)
+    _check(lib().sdgso_kmeans(_p(p), len(p), int(k), int(seed), int(iterations), _p(c), _p(g, _I)))
+    return c, g
+
+
+def starting_poses(receptor, rec_xyz, lig_xyz, rec_translation, swarms, glowworms,
+                   swarm_radius=10.0, start_points_seed=324324, anm_seed=324324, n_modes_rec=0,
+                   n_modes_lig=0):
+    """Swarm centres [swarms][3] and poses [swarms][glowworms][P] around the
+    receptor surface (coordinates before centring)."""
+    rx = np.ascontiguousarray(rec_xyz, dtype=np.float64)
+    lx = np.ascontiguousarray(lig_xyz, dtype=np.float64)
+    t = np.ascontiguousarray(rec_translation, dtype=np.float64)
+    P = 7 + n_modes_rec + n_modes_lig
+    c = np.zeros((swarms, 3))
+    pos = np.zeros((swarms, glowworms, P))
+    _check(lib().sdgso_starting_poses(receptor.h, _p(rx), len(lx), _p(lx), _p(t), int(swarms),
+                                      int(glowworms), float(swarm_radius), int(start_points_seed),
+                                      int(anm_seed), int(n_modes_rec), int(n_modes_lig), _p(c), _p(pos)))
+    return c, pos
 
 
 def reference_point(xyz, tolerance=0.01):

@@ -257,6 +257,7 @@ def ref():
         R.ref_select.restype = ctypes.c_int
         R.ref_select.argtypes = [ctypes.c_double, _I, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         R.ref_std_normal.argtypes = [ctypes.c_uint, ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
+        R.ref_kmeans.argtypes = [_D, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D, _I]
         R.ref_rmsd.restype = ctypes.c_double
         R.ref_rmsd.argtypes = [_D, _D, ctypes.c_int]
         R.ref_mvee.argtypes = [_D, ctypes.c_int, ctypes.c_double, _D]

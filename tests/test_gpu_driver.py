@@ -128,10 +128,8 @@ def test_lightdock_pdb_inputs(tmp_path, use_anm):
     table = sdgso.synth_table(7)
     cx = sdgso.Complex(rx, rs.dfire_types(), lx, ls.dfire_types(), table, ref, rm, lm,
                        rs.nm_mask(), ls.nm_mask())
-    # the driver's radii: sqrt(x*x + y*y + z*z) per atom, in that order
-    rrad = np.max(np.sqrt(rx[:, 0] * rx[:, 0] + rx[:, 1] * rx[:, 1] + rx[:, 2] * rx[:, 2]))
-    lrad = np.max(np.sqrt(lx[:, 0] * lx[:, 0] + lx[:, 1] * lx[:, 1] + lx[:, 2] * lx[:, 2]))
-    pos = sdgso.synth_poses(11, S, G, nm, nm, rrad + 0.5 * lrad, 5.0)
+    _, pos = sdgso.starting_poses(rs, rs.coords(), ls.coords(), rs.centre(), S, G, 5.0, 11,
+                                  324324, nm, nm)
     run = sdgso.Run(cx, S, G, pos, sdgso.stream(0, S, steps, G), steps, form=sdgso.FP32_GUARD)
     run.steps(steps, graph=True)
     got = run.read()

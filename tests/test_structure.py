@@ -10,6 +10,7 @@ complex sizes of the batched workload).  Those comparisons run where the
 reference sources are present and are skipped elsewhere.
 """
 import glob
+import math
 import os
 
 import numpy as np
@@ -254,3 +255,108 @@ def test_anm_hessian_properties():
     # a rigid translation is a zero mode exactly
     t = np.tile([0.3, -1.2, 0.7], 30)
     assert np.allclose(H @ t, 0.0, atol=1e-12)
+
+
+# ---------------------------------------------------------------- starting poses
+
+PDB_FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "pdb")
+
+
+@need_ref
+def test_kmeans_matches_reference():
+    rng = np.random.default_rng(9)
+    for n, k, it in ((500, 7, 100), (2000, 40, 30), (64, 63, 5)):
+        pts = rng.normal(size=(n, 3)) * 12.0
+        c, g = sdgso.kmeans(pts, k, seed=324324, iterations=it)
+        wc = np.zeros((k, 3))
+        wg = np.zeros(n, np.int32)
+        O.ref().ref_kmeans(O._p(pts), n, k, 324324, it, O._p(wc), O._p(wg, O._I))
+        assert np.array_equal(c, wc) and np.array_equal(g, wg)
+
+
+def _sphere_points(n):
+    """cal_anm.cc:977-989"""
+    inc = math.pi * (3.0 - math.sqrt(5.0))
+    off = 2.0 / n
+    out = np.zeros((n, 3))
+    for i in range(n):
+        y = i * off - 1.0 + (off / 2.0)
+        r = math.sqrt(1 - y * y)
+        phi = i * inc
+        out[i] = (math.cos(phi) * r, y, math.sin(phi) * r)
+    return out
+
+
+def _dist3(a, b):
+    d = a - b
+    return np.sqrt((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2])
+
+
+@need_ref
+def test_starting_poses_restated():
+    """calculate_surface_point / calculate_start_pos restated with the
+    reference's k-means, the oracle's mt19937 stream and libstdc++ normals."""
+    rs = sdgso.Structure.read_pdb(os.path.join(PDB_FIX, "2X9A_r_u.pdb"))
+    ls = sdgso.Structure.read_pdb(os.path.join(PDB_FIX, "2X9A_l_u.pdb"))
+    rx, lx, t = rs.coords(), ls.coords(), rs.centre()
+    S, G, radius, nm = 12, 9, 10.0, 3
+    centres, pos = sdgso.starting_poses(rs, rx, lx, t, S, G, radius, 1234, 77, nm, nm)
+    # ligand diameter -> surface distance
+    diam = max(_dist3(lx[i], lx[i + 1:]).max() for i in range(len(lx) - 1))
+    sd = 10.0 if diam < 20.0 else diam / 4.0
+    surface = {"ARG", "ASN", "ASP", "ASX", "CSO", "GLN", "GLU", "GLX", "GLY", "HID", "HIE", "HIP",
+               "IS", "HSD", "HSE", "HSP", "LYS", "PHD", "PRO", "PTR", "SEP", "SER", "THR", "TPO",
+               "TYR", "HIS"}
+    surf = np.array([rx[i] for i in range(rs.n_atoms) if rs.atom(i)[1] in surface])
+
+    def ref_kmeans(p, k, it):
+        c = np.zeros((k, 3))
+        g = np.zeros(len(p), np.int32)
+        p = np.ascontiguousarray(p)
+        O.ref().ref_kmeans(O._p(p), len(p), k, 324324, it, O._p(c), O._p(g, O._I))
+        return c
+
+    cent = ref_kmeans(surf, S, 100)
+    sph = _sphere_points(100)
+    samp = np.array([[sph[j] * sd + cent[i] for j in range(100)] for i in range(S)])
+    drop = np.zeros((S, 100), bool)
+    for i in range(S):
+        for j in range(S):
+            if j != i and _dist3(cent[i], cent[j]) <= 20.0:
+                drop[i] |= _dist3(samp[i], cent[j]) <= sd
+    kept = samp[~drop]
+    outer = np.array([p for p in kept if not np.any(_dist3(rx, p) <= 3.0)])
+    want_c = ref_kmeans(outer, S, 500) + t
+    assert np.array_equal(centres, want_c)
+    # poses from the mt19937 stream and libstdc++ normals
+    u = iter(O.stream(1234, 1, 1, 20 * S * G).ravel())
+    nrm = np.zeros(2 * nm * S * G)
+    O.ref().ref_std_normal(77, len(nrm), 0.0, 1.0, O._p(nrm))
+    nrm = iter(nrm)
+    for i in range(S):
+        for j in range(G):
+            while True:
+                x, y, z = ((2 * next(u) - 1) * radius for _ in range(3))
+                if x * x + y * y + z * z <= radius * radius:
+                    break
+            u1, u2, u3 = next(u), next(u), next(u)
+            q = [math.sqrt(1 - u1) * math.sin(2 * math.pi * u2), math.sqrt(1 - u1) * math.cos(2 * math.pi * u2),
+                 math.sqrt(u1) * math.sin(2 * math.pi * u3), math.sqrt(u1) * math.cos(2 * math.pi * u3)]
+            want = [want_c[i][0] + x, want_c[i][1] + y, want_c[i][2] + z] + q + [next(nrm) for _ in range(2 * nm)]
+            assert np.array_equal(pos[i, j], np.array(want)), (i, j)
+    rs.close()
+    ls.close()
+
+
+def test_starting_poses_properties():
+    rs = sdgso.Structure.read_pdb(os.path.join(PDB_FIX, "2X9A_r_u.pdb"))
+    ls = sdgso.Structure.read_pdb(os.path.join(PDB_FIX, "2X9A_l_u.pdb"))
+    c, pos = sdgso.starting_poses(rs, rs.coords(), ls.coords(), rs.centre(), 20, 15, 8.0)
+    assert pos.shape == (20, 15, 7)
+    assert np.allclose(np.linalg.norm(pos[:, :, 3:7], axis=2), 1.0, atol=1e-12)
+    assert np.all(np.linalg.norm(pos[:, :, :3] - c[:, None, :], axis=2) <= 8.0 + 1e-12)
+    # centres lie outside the centred receptor's atoms on average distance
+    rc = rs.coords() + rs.centre()
+    assert np.min([np.min(np.linalg.norm(rc - x, axis=1)) for x in c]) > 1.0
+    with pytest.raises(sdgso.SdgsoError):
+        sdgso.starting_poses(rs, rs.coords(), ls.coords(), rs.centre(), 10**5, 2)
