@@ -191,7 +191,8 @@ def main():
                     help="FP32 scoring-kernel variant to use instead of the autotuner's choice")
     ap.add_argument("--profile-step", action="store_true",
                     help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
-                         "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE")
+                         "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE, "
+                         "then the random-gather probe")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the GSO path has no multi-GPU form in this build "
@@ -217,6 +218,8 @@ def main():
         run.steps(1)
         run.sync()
         log("profile step done: %d glowworms" % (sum(wl["n_swarms"]) * wl["G"]))
+        # the random-gather ceiling kernel, for counters to compare against
+        log("gather probe: %.3g gathers/s" % sdgso.probe_gather())
         return
 
     t0 = time.perf_counter()

@@ -38,6 +38,8 @@
 //
 // Lanes are combined in FP64 and written as one partial per (glowworm,
 // block); finish32 adds a glowworm's partials in block order in FP64.
+#include <type_traits>
+
 #include "dev_common.h"
 
 namespace sd {
@@ -135,14 +137,14 @@ struct WaveCtx {
 
 // Candidate tiles from tile t0 on: 64 sphere tests per instruction, listed
 // in LDS (at most `cap` entries); advances t0 past the tiles tested.
-__device__ __forceinline__ int scan_candidates(const DevComplex &c, const WaveCtx &x,
+__device__ __forceinline__ int scan_candidates(int ntiles, const float4 *tiles, const WaveCtx &x,
                                                unsigned short *ct, int &t0, int cap) {
     int n = 0;
-    for (; t0 < c.ntiles && n <= cap - 64; t0 += 64) {
+    for (; t0 < ntiles && n <= cap - 64; t0 += 64) {
         const int t = t0 + x.lane;
         bool ok = false;
-        if (t < c.ntiles) {
-            const float4 tl = gld(c.tiles, t);
+        if (t < ntiles) {
+            const float4 tl = gld(tiles, t);
             const float tx = tl.x - x.fc[0], ty = tl.y - x.fc[1], tz = tl.z - x.fc[2];
             const float lim = x.reach + tl.w;
             ok = tx * tx + ty * ty + tz * tz <= lim * lim;
@@ -156,6 +158,11 @@ __device__ __forceinline__ int scan_candidates(const DevComplex &c, const WaveCt
     }
     wave_sync_lds();
     return n;
+}
+
+__device__ __forceinline__ int scan_candidates(const DevComplex &c, const WaveCtx &x,
+                                               unsigned short *ct, int &t0, int cap) {
+    return scan_candidates(c.ntiles, c.tiles, x, ct, t0, cap);
 }
 
 // Set-up common to the variants: pose, block sphere, ligand atom, receptor
@@ -575,16 +582,26 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 // Near receptor atoms are collected across the visited tiles (their slots in a
 // pending list) and displaced 64 at a time, one per lane, so every lane of
 // the displacement has an atom; the displaced atoms are packed into LDS and
-// the wave walks them with broadcast LDS reads, four per round.  Each lane pre-tests its ligand atom with the FP32
-// form V = (|r|^2 + |l|^2) - 2 r.l in A^2, and the pairs that pass are queued
-// as one word (packed receptor, ligand lane); the exact fixed-point d2 is
-// formed in the evaluate phase, where every lane has a pair.
+// the wave walks them with broadcast LDS reads, four per round.  Each lane
+// pre-tests its ligand atom with the FP32 form W = |r|^2 - 2 r.l against
+// T - |l|^2 (A^2), and the pairs that pass are queued as one word: the LDS
+// byte offsets of the packed receptor atom (high half) and of the ligand atom
+// (low half).  The exact fixed-point d2 is formed in the evaluate phase, 64
+// queued pairs per round, every lane with a pair (a partial round is padded
+// with a far-away atom).  A round's table gather is a buffer load whose offset
+// is past the table's end for a pair that does not count (the load returns 0
+// without a memory access); its value is added kUnroll rounds later, so that
+// kUnroll rounds of gathers are in flight per wave, also across the pre-test
+// work between two flushes.  Only whole groups of kUnroll rounds are
+// evaluated until the wave's last batch; the packed atoms are double-buffered
+// so that the pairs left over from one batch wait in the queue for the next.
 //
 // Pre-test bound: with u = 2^-24, every pair that can count (d2 < 225 + band)
 // has |l| <= rho - 15 and |r| <= rho, rho the wave's reach; converting the
-// fixed-point values to float, squaring and the four rounded adds of V stay
-// within 28 u rho^2 of the exact d2, so V <= 225 + 2 band + 2^-18 rho^2 + 10^-3
-// keeps every such pair (extra pairs are rejected by the exact evaluation).
+// fixed-point values to float, squaring, the three rounded FMAs of W and the
+// rounded T - |l|^2 stay within 28 u rho^2 of the exact d2, so
+// T = 225 + 2 band + 2^-18 rho^2 + 10^-3 keeps every such pair (extra pairs
+// are rejected by the exact evaluation).
 //
 // MFMA pre-test (kMfma): V of 32 x 32 pairs is one v_mfma_f32_32x32x8f16 with
 // rows (x, y, z, n_hi | n_lo, 1, 1, 0) of the packed receptor atoms and
@@ -597,27 +614,61 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 // Rows past the batch and padding ligand lanes carry a norm of 60000, so they
 // never pass; a wave whose atoms could exceed the half range (|r| >= 250 A)
 // takes the FP32 pre-test.
-constexpr int kQueueB = 512;
-constexpr int kQueueBM = 1280;  // MFMA pre-test: room for a 32 x 32 group over the flush mark
-constexpr int kCandB = 128;  // candidate-list chunk of the broadcast variant
+// Queue room: kUnroll rounds of 64 pairs left over from a flush, plus the
+// pairs of four pre-tested atoms (or of one MFMA 32 x 32 group).
+template <int kUnroll, bool kMfma>
+constexpr int queue_len() { return 64 * kUnroll + (kMfma ? 1024 : 256); }
+constexpr int kCandB = 128;     // candidate-list chunk of the broadcast variant
+constexpr int kRtRows = 2 * 64 + 1;        // two batches of packed atoms and one far-away atom
+constexpr unsigned kNoLoad = 0x80000000u;  // buffer offset past the table: the load returns 0
 
-template <int kUnroll, int kMinWaves, bool kMfma>
-__global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(ScoreArgs a) {
-    constexpr int kQueue = kMfma ? kQueueBM : kQueueB;
-    __shared__ int4 s_rt[kWaves][64];          // packed near receptor atoms: fixed point + row
-    __shared__ float4 s_rf[kWaves][64];        //   the same in A: x, y, z, |r|^2
-    __shared__ unsigned short s_rs[kWaves][64];  //   their receptor slots (tile*64 + atom)
-    __shared__ unsigned short s_pd[kWaves][2 * 64];  // near slots waiting for displacement
-    __shared__ int4 s_lg[kWaves][64];          // the block's ligand atoms: fixed point + type*20
-    __shared__ int s_q[kWaves][kQueue];        // queued pairs: (packed receptor << 6) | lane
-    __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
-    __shared__ unsigned short s_ct[kWaves][kCandB];  // candidate tiles, scanned in chunks
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wv = blockIdx.x * kWaves + w;
+// f(std::integral_constant<int, I>) for I = I0 .. N-1, unrolled.
+template <int I, int N, class F>
+__device__ __forceinline__ void static_for(F &&f) {
+    if constexpr (I < N) {
+        f(std::integral_constant<int, I>{});
+        static_for<I + 1, N>(f);
+    }
+}
+
+// A pointer the whole wave holds (scalar registers).
+template <class T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    const unsigned long long v = (unsigned long long)p;
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return (T *)(((unsigned long long)hi << 32) | lo);
+}
+
+template <int kUnroll, int kMinWaves, bool kMfma, int kWpb>
+__global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArgs a) {
+    constexpr int kQueue = queue_len<kUnroll, kMfma>();
+    constexpr int kGroup = 64 * kUnroll;  // pairs of one group of rounds
+    __shared__ int4 s_rt[kWpb][kRtRows];     // packed near receptor atoms: fixed point + row
+    __shared__ float4 s_rf[kWpb][64];        //   this batch's atoms in A: x, y, z, |r|^2
+    __shared__ unsigned short s_rs[kWpb][2 * 64];  // receptor slots of the packed atoms
+    __shared__ unsigned short s_pd[kWpb][2 * 64];  // near slots waiting for displacement
+    __shared__ int4 s_lg[kWpb][64];          // the block's ligand atoms: fixed point + type*20
+    __shared__ int s_q[kWpb][kQueue];        // queued pairs (LDS offsets of both atoms)
+    __shared__ float s_ef[kWpb][SDGSO_MAX_MODES];
+    __shared__ unsigned short s_ct[kWpb][kCandB];  // candidate tiles, scanned in chunks
+    const int w = kWpb == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int wv = blockIdx.x * kWpb + w;
     if (wv >= a.n_work) return;
-    const int gid = a.work[2 * wv];
-    const int b = a.work[2 * wv + 1];
-    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
+    const int gid = __builtin_amdgcn_readfirstlane(a.work[2 * wv]);
+    const int b = __builtin_amdgcn_readfirstlane(a.work[2 * wv + 1]);
+    const DevComplex &c = a.cx[__builtin_amdgcn_readfirstlane(a.swarm_cx[gid / a.G])];
+    // the fields the tile visits and the evaluation read, in scalar registers
+    const int ntiles = __builtin_amdgcn_readfirstlane(c.ntiles);
+    const int nmr = __builtin_amdgcn_readfirstlane(c.nmr);
+    const int mstride4 = __builtin_amdgcn_readfirstlane(c.rmod_stride / 4);
+    const float4 *tiles = uniform_ptr(c.tiles);
+    const float4 *rpos32 = uniform_ptr(c.rpos32);
+    const double *rrel4 = uniform_ptr(c.rrel4);
+    const float4 *rmodA = uniform_ptr((const float4 *)c.rmodA);
+    const int *rtype_s = uniform_ptr(c.rtype_s);
+    const __amdgpu_buffer_rsrc_t tbl = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)uniform_ptr(c.table), (short)0, kTableLen * 4, 0x00020000);
     int4 *rt = s_rt[w];
     float4 *rf = s_rf[w];
     unsigned short *rs = s_rs[w];
@@ -630,14 +681,22 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
     wave_setup(a, c, gid, b, ef, ct, x, kCandB);
     const int lane = x.lane;
     lg[lane] = x.L;
+    if (lane == 0) rt[2 * 64] = make_int4(kFar, kFar, kFar, 0);
     wave_sync_lds();
+    // queue words: LDS byte offsets from s_rt[0][0] (high half) and s_lg[0][0] (low half)
+    const char *rt_base = (const char *)&s_rt[0][0];
+    const char *lg_base = (const char *)&s_lg[0][0];
+    const unsigned rt0 = (unsigned)(w * kRtRows) * 16u;
+    const unsigned lane_lo = (unsigned)(w * 64 + lane) * 16u;
+    const int null_tag = (int)(((rt0 + 2 * 64 * 16u) << 16) | (unsigned)(w * 64) * 16u);
     const float bandq = a.band * kQ2;
     const float thr = 225.0f + 2.0f * a.band + 0x1p-18f * x.reach * x.reach + 1e-3f;
     const float reach2 = x.reach * x.reach;
-    // this lane's pre-test operands: -2 l and |l|^2 (padding lanes never pass)
+    // this lane's pre-test operands: -2 l and T - |l|^2 (padding lanes never pass)
     const float lx = (float)x.L.x * kInvQ, ly = (float)x.L.y * kInvQ, lz = (float)x.L.z * kInvQ;
     const float m2x = -2.f * lx, m2y = -2.f * ly, m2z = -2.f * lz;
-    const float nl = x.lig >= 0 ? __builtin_fmaf(lz, lz, __builtin_fmaf(ly, ly, lx * lx)) : 1e30f;
+    const float tl = x.lig >= 0 ? thr - __builtin_fmaf(lz, lz, __builtin_fmaf(ly, ly, lx * lx))
+                                : -1e30f;
     // MFMA pre-test: B operands of the two 32-atom column groups, and the threshold
     const bool mfma_ok = kMfma && x.rmax < 250.0f;
     const float thr16 = 225.0f + 2.0f * a.band + 0x1p-8f * x.reach * (x.reach - 15.0f) +
@@ -663,78 +722,96 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
     }
 
     Kahan acc;
-    int npairs = 0;
+    int npairs = 0;          // in-cutoff pairs evaluated by this lane
     unsigned int nguard = 0;
     int qn = 0;
+    int old = 0;             // queued pairs of the previous batch (front of the queue)
+    int p = 0;               // buffer of the batch being packed
     long long n_near = 0, n_queued = 0;
-    // table values of the previous evaluate round, still in flight: they are
-    // added one round later (or at the end), so their latency overlaps work
-    float pv[kUnroll];
-    bool ptake[kUnroll];
+    // table values in flight: slot j holds the load of the last round that used it
+    float ring[kUnroll];
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-        pv[u] = 0.f;
-        ptake[u] = false;
-    }
-    auto drain = [&]() {
-#pragma unroll
-        for (int u = 0; u < kUnroll; u++) {
-            acc.add(pv[u], ptake[u]);
-            npairs += ptake[u] ? 1 : 0;
-            ptake[u] = false;
+    for (int j = 0; j < kUnroll; j++) ring[j] = 0.f;
+
+    // One round: the 64 queued pairs from entry e0, table load into ring slot j
+    // (whose previous load is added first).
+    auto round = [&](auto jc, int e0) {
+        constexpr int j = decltype(jc)::value;
+        const int tag = qq[e0 + lane];
+        const int4 r = *(const int4 *)(rt_base + ((unsigned)tag >> 16));
+        const int4 l = *(const int4 *)(lg_base + (tag & 0xffff));
+        const PairClass pc = classify(d2_fixed(l, r), bandq);
+        const unsigned off = pc.take ? 4u * (unsigned)(r.w + l.w + pc.bin) : kNoLoad;
+        acc.add(ring[j], true);
+        ring[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tbl, off, 0, 0));
+        npairs += pc.take ? 1 : 0;
+        if (pc.near) {
+            const int li = (int)((unsigned)(tag & 0xffff) >> 4) - w * 64;
+            const int ki = (int)((unsigned)tag >> 20) - w * kRtRows;
+            const float2 g2 = guard_pair64(c, x.pose, gld(c.lperm, x.b * kBlock + li),
+                                           gld(c.rperm, (int)rs[ki]));
+            acc.add(g2.x, g2.y != 0.f);
+            npairs += g2.y != 0.f ? 1 : 0;
+            nguard++;
         }
     };
-    auto flush = [&]() {
+    // Whole groups of kUnroll rounds: every slot is reloaded in every group, so
+    // a slot's load has the other kUnroll - 1 loads behind it when it is added.
+    auto eval_groups = [&](int ng) {
+        for (int g = 0; g < ng; g++)
+            static_for<0, kUnroll>([&](auto jc) { round(jc, kGroup * g + 64 * decltype(jc)::value); });
+    };
+    // Rounds 0..nround-1, for the last flush of a batch or of the wave.
+    auto eval_tail = [&](int nround) {
+        for (int r0 = 0; r0 < nround; r0 += kUnroll)
+            static_for<0, kUnroll>([&](auto jc) {
+                if (r0 + decltype(jc)::value < nround) round(jc, 64 * (r0 + decltype(jc)::value));
+            });
+    };
+    // Evaluate the whole groups of the queue; the rest (< kGroup) moves to the front.
+    auto flush_groups = [&]() {
+        const int ng = qn / kGroup;
+        if (ng == 0) return;
         wave_sync_lds();
-        n_queued += qn;
-        for (int base = 0; base < qn; base += 64 * kUnroll) {
-            float v[kUnroll];
-            bool take[kUnroll], near[kUnroll];
-            int tag[kUnroll];
+        eval_groups(ng);
+        const int rem = qn - kGroup * ng;
+        int v[kUnroll];
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const int e = base + u * 64 + lane;
-                const bool valid = e < qn;
-                const int raw = qq[min(e, kQueue - 1)];
-                tag[u] = valid ? raw : 0;
-                const int4 r = rt[tag[u] >> 6];
-                const int4 l = lg[tag[u] & 63];
-                const PairClass pc = classify(d2_fixed(l, r), bandq);
-                near[u] = valid && pc.near;
-                take[u] = valid && pc.take;
-                v[u] = gld(c.table, take[u] ? r.w + l.w + pc.bin : kZeroIdx);
-            }
-            drain();
+        for (int j = 0; j < kUnroll; j++)
+            v[j] = 64 * j + lane < rem ? qq[kGroup * ng + 64 * j + lane] : 0;
+        wave_sync_lds();
 #pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                pv[u] = v[u];
-                ptake[u] = take[u];
-                if (near[u]) {
-                    const float2 g2 = guard_pair64(c, x.pose,
-                                                   gld(c.lperm, x.b * kBlock + (tag[u] & 63)),
-                                                   gld(c.rperm, (int)rs[tag[u] >> 6]));
-                    acc.add(g2.x, g2.y != 0.f);
-                    npairs += g2.y != 0.f ? 1 : 0;
-                    nguard++;
-                }
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
+        for (int j = 0; j < kUnroll; j++)
+            if (64 * j + lane < rem) qq[64 * j + lane] = v[j];
+        qn = rem;
+        old = max(0, old - kGroup * ng);
+    };
+    // Evaluate everything (after flush_groups: qn < kGroup), padding the last round.
+    auto flush_all = [&]() {
+        if (qn == 0) return;
+        const int pad = (-qn) & 63;
+        if (lane < pad) qq[qn + lane] = null_tag;
+        wave_sync_lds();
+        eval_tail((qn + pad) >> 6);
+        wave_sync_lds();
         qn = 0;
+        old = 0;
     };
 
     // Displace the pending atoms pd[0..n) (n <= 64, one per lane), pack them
-    // into LDS, queue their pairs that pass the pre-test and evaluate them
-    // (the queue is empty between batches: its entries index the packed atoms).
+    // into buffer p, queue their pairs that pass the pre-test and evaluate the
+    // whole rounds.  Pairs of the previous batch (buffer 1-p) that are still
+    // queued at the end are evaluated too, since the next batch overwrites it.
     auto batch = [&](int n) {
+        old = qn;
         if (lane < n) {
             const int slot = pd[lane];
-            const float4 tl = gld(c.tiles, slot / kTile);
-            const double2 rxy = gld2(c.rrel4, 2 * (size_t)slot);
-            const double rz = gld(c.rrel4, 4 * (size_t)slot + 2);
+            const float4 tc = gld(tiles, slot / kTile);
+            const double2 rxy = gld2(rrel4, 2 * (size_t)slot);
+            const double rz = gld(rrel4, 4 * (size_t)slot + 2);
             float Dx = 0.f, Dy = 0.f, Dz = 0.f;
-            const float4 *MA = (const float4 *)c.rmodA + (size_t)slot * (c.rmod_stride / 4);
-            const int ngroups = (c.nmr + 3) / 4;  // groups of four modes: twelve floats, three loads
+            const float4 *MA = rmodA + (size_t)slot * mstride4;
+            const int ngroups = (nmr + 3) / 4;  // groups of four modes: twelve floats, three loads
             for (int g0 = 0; g0 < ngroups; g0 += 3) {
                 // up to three groups per round, loads issued together (a group past
                 // the last one re-reads the last; its products are not added)
@@ -750,7 +827,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
 #pragma unroll
                     for (int k = 0; k < 4; k++) {
                         const int m = 4 * (g0 + j) + k;
-                        if (m < c.nmr) {
+                        if (m < nmr) {
                             Dx = __builtin_fmaf(mv[3 * k], ef[m], Dx);
                             Dy = __builtin_fmaf(mv[3 * k + 1], ef[m], Dy);
                             Dz = __builtin_fmaf(mv[3 * k + 2], ef[m], Dz);
@@ -758,19 +835,19 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                     }
                 }
             }
-            const int4 R = make_int4(qround(((double)tl.x - x.cb[0]) + rxy.x + (double)Dx),
-                                     qround(((double)tl.y - x.cb[1]) + rxy.y + (double)Dy),
-                                     qround(((double)tl.z - x.cb[2]) + rz + (double)Dz),
-                                     gld(c.rtype_s, slot));
+            const int4 R = make_int4(qround(((double)tc.x - x.cb[0]) + rxy.x + (double)Dx),
+                                     qround(((double)tc.y - x.cb[1]) + rxy.y + (double)Dy),
+                                     qround(((double)tc.z - x.cb[2]) + rz + (double)Dz),
+                                     gld(rtype_s, slot));
             const float fx = (float)R.x * kInvQ, fy = (float)R.y * kInvQ, fz = (float)R.z * kInvQ;
-            rt[lane] = R;
+            rt[64 * p + lane] = R;
             rf[lane] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
-            rs[lane] = (unsigned short)slot;
+            rs[64 * p + lane] = (unsigned short)slot;
         }
         wave_sync_lds();
+        const unsigned hib = rt0 + 64u * 16u * (unsigned)p;  // offset of this buffer's atom 0
         if (mfma_ok) {
             // lane l: row 8*(v/4) + 4*(l/32) + v%4 of accumulator v, column l%32
-            const int lanetag = ((4 * (lane >> 5)) << 6) | (lane & 31);
             for (int h = 0; h * 32 < n; h++) {
                 const int ri = 32 * h + (lane & 31);
                 const float4 r = rf[ri];
@@ -791,11 +868,13 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
 #pragma unroll 1
                 for (int g = 0; g < 2; g++) {
                     if (32 * g >= x.nblk) break;
-                    if (qn > kQueue - 1024) flush();
+                    if (qn > kQueue - 1024) flush_groups();
                     floatx16 d = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
                                   0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
                     d = __builtin_amdgcn_mfma_f32_32x32x8f16(av, g ? bop[1] : bop[0], d, 0, 0, 0);
-                    const int tagb = ((32 * h) << 6) + 32 * g + lanetag;
+                    const unsigned tagb =
+                        ((hib + 16u * (unsigned)(32 * h + 4 * (lane >> 5))) << 16) |
+                        ((unsigned)(w * 64 + 32 * g + (lane & 31)) * 16u);
 #pragma unroll
                     for (int v = 0; v < 16; v++) {
                         const bool hit = d[v] <= thr16;
@@ -805,79 +884,94 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                                 qq[qn + __builtin_amdgcn_mbcnt_hi(
                                             (unsigned)(m >> 32),
                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
-                                    tagb + ((8 * (v >> 2) + (v & 3)) << 6);
+                                    (int)(tagb + ((16u * (unsigned)(8 * (v >> 2) + (v & 3))) << 16));
                             qn += __popcll(m);
+                            n_queued += __popcll(m);
                         }
                     }
                 }
             }
-            if (qn) flush();
-            return;
-        }
-        // four packed atoms per round: their broadcast reads are in flight together
-        for (int i0 = 0; i0 < n; i0 += 4) {
-            float4 r[4];
+        } else {
+            // four packed atoms per round: their broadcast reads are in flight together
+            for (int i0 = 0; i0 < n; i0 += 4) {
+                float4 r[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) r[j] = rf[min(i0 + j, 63)];
-            unsigned long long m[4];
-            bool in[4];
+                for (int j = 0; j < 4; j++) r[j] = rf[min(i0 + j, 63)];
+                unsigned long long m[4];
+                bool in[4];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float d2a = __builtin_fmaf(
-                    r[j].x, m2x,
-                    __builtin_fmaf(r[j].y, m2y, __builtin_fmaf(r[j].z, m2z, r[j].w + nl)));
-                in[j] = d2a <= thr;
-                m[j] = i0 + j < n ? __ballot(in[j]) : 0ull;
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (m[j]) {
-                    if (in[j])
-                        qq[qn + __builtin_amdgcn_mbcnt_hi(
-                                    (unsigned)(m[j] >> 32),
-                                    __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
-                            ((i0 + j) << 6) | lane;
-                    qn += __popcll(m[j]);
+                for (int j = 0; j < 4; j++) {
+                    const float wv2 = __builtin_fmaf(
+                        r[j].x, m2x, __builtin_fmaf(r[j].y, m2y, __builtin_fmaf(r[j].z, m2z, r[j].w)));
+                    in[j] = wv2 <= tl;
+                    m[j] = i0 + j < n ? __ballot(in[j]) : 0ull;
                 }
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
+                    if (m[j]) {
+                        if (in[j])
+                            qq[qn + __builtin_amdgcn_mbcnt_hi(
+                                        (unsigned)(m[j] >> 32),
+                                        __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
+                                (int)(((hib + 16u * (unsigned)(i0 + j)) << 16) | lane_lo);
+                        qn += __popcll(m[j]);
+                        n_queued += __popcll(m[j]);
+                    }
+                }
+                if (qn > kQueue - 4 * 64) flush_groups();
             }
-            if (qn > kQueue - 4 * 64) flush();
         }
-        if (qn) flush();
+        flush_groups();
+        if (old > 0) flush_all();
+        p ^= 1;
     };
 
     // Visit the candidate tiles (chunks of kCandB): near test of every slot
     // against the block sphere; near slots join the pending list.
     int npd = 0;
     long long n_cand = 0;
+    // near test of this lane's atom of tile t (centre tc, atom pp); near slots are appended
+    auto visit = [&](int t, float4 tc, float4 pp) {
+        const float ax = pp.x + (tc.x - x.fc[0]);
+        const float ay = pp.y + (tc.y - x.fc[1]);
+        const float az = pp.z + (tc.z - x.fc[2]);
+        const bool nearb = ax * ax + ay * ay + az * az <= reach2;
+        const unsigned long long m = __ballot(nearb);
+        if (!m) return;
+        if (nearb)
+            pd[npd + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
+                                               __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                (unsigned short)(t * kTile + lane);
+        npd += __popcll(m);
+        n_near += __popcll(m);
+    };
     for (;;) {
-        n_cand += x.ncand;
-        int t_next = 0;
-        float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f), p_next = tl_next;
-        if (x.ncand) {
-            t_next = __builtin_amdgcn_readfirstlane((int)ct[0]);
-            tl_next = gld(c.tiles, t_next);
-            p_next = gld(c.rpos32, t_next * kTile + lane);
-        }
-        for (int k = 0; k < x.ncand; k++) {
-            const int t = t_next;
-            const float4 tl = tl_next, p = p_next;
-            if (k + 1 < x.ncand) {
-                t_next = __builtin_amdgcn_readfirstlane((int)ct[k + 1]);
-                tl_next = gld(c.tiles, t_next);
-                p_next = gld(c.rpos32, t_next * kTile + lane);
+        const int nc = x.ncand;
+        n_cand += nc;
+        // the chunk's tile list in registers; tile k is lane k of c0 / c1
+        const int c0 = ct[lane], c1 = ct[64 + lane];
+        auto tile_of = [&](int k) {
+            k = min(k, nc - 1);
+            return k < 64 ? __builtin_amdgcn_readlane(c0, k) : __builtin_amdgcn_readlane(c1, k - 64);
+        };
+        for (int k = 0; k < nc;) {
+            // two tiles in flight: A is visited while B loads, then the roles swap;
+            // a full pending list leaves the loop for the batch (one call site)
+            int tA = tile_of(k);
+            float4 tcA = gld(tiles, tA), pA = gld(rpos32, tA * kTile + lane);
+            for (;;) {
+                const int tB = tile_of(k + 1);
+                const float4 tcB = gld(tiles, tB), pB = gld(rpos32, tB * kTile + lane);
+                visit(tA, tcA, pA);
+                k++;
+                if (npd >= 64 || k >= nc) break;
+                tA = tile_of(k + 1);
+                tcA = gld(tiles, tA);
+                pA = gld(rpos32, tA * kTile + lane);
+                visit(tB, tcB, pB);
+                k++;
+                if (npd >= 64 || k >= nc) break;
             }
-            const float ax = p.x + (tl.x - x.fc[0]);
-            const float ay = p.y + (tl.y - x.fc[1]);
-            const float az = p.z + (tl.z - x.fc[2]);
-            const bool nearb = ax * ax + ay * ay + az * az <= reach2;
-            const unsigned long long m = __ballot(nearb);
-            if (!m) continue;
-            if (nearb)
-                pd[npd + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
-                                                   __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
-                    (unsigned short)(t * kTile + lane);
-            npd += __popcll(m);
-            n_near += __popcll(m);
             if (npd >= 64) {
                 wave_sync_lds();
                 batch(64);
@@ -889,15 +983,17 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32b_kernel(Sc
                 npd = rem;
             }
         }
-        if (x.t_scan >= c.ntiles) break;
+        if (x.t_scan >= ntiles) break;
         __builtin_amdgcn_wave_barrier();  // every lane has read the list before it is rewritten
-        x.ncand = scan_candidates(c, x, ct, x.t_scan, kCandB);
+        x.ncand = scan_candidates(ntiles, tiles, x, ct, x.t_scan, kCandB);
     }
     if (npd) {
         wave_sync_lds();
         batch(npd);
     }
-    drain();
+    flush_all();
+#pragma unroll
+    for (int j = 0; j < kUnroll; j++) acc.add(ring[j], true);
     write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued);
 }
 
@@ -935,21 +1031,23 @@ __global__ void advance_kernel(int *k) {
     if (threadIdx.x == 0) *k += 1;
 }
 
-// Variant table of the FP32 scoring kernel: filter x (gathers in flight,
-// min waves/SIMD).
+// Variant table of the FP32 scoring kernel: filter x (rounds of table
+// gathers in flight, min waves/SIMD).
 namespace {
 struct Variant {
     const char *name;
     void (*kernel)(ScoreArgs);
+    int waves;  // waves per workgroup
 };
 const Variant kVariants[] = {
-    {"bcast-mfma16/unroll4/occ4", score32b_kernel<4, 4, true>},
-    {"bcast-mfma16/unroll2/occ4", score32b_kernel<2, 4, true>},
-    {"bcast/unroll4/occ4", score32b_kernel<4, 4, false>},
-    {"bcast/unroll2/occ5", score32b_kernel<2, 5, false>},
-    {"readlane/unroll4/occ4", score32r_kernel<4, 4>},
-    {"readlane/unroll4/occ3", score32r_kernel<4, 3>},
-    {"mfma/unroll4/occ3", score32m_kernel<4, 3>},
+    {"bcast-mfma16/ring3/occ4", score32b_kernel<3, 4, true, 1>, 1},
+    {"bcast-mfma16/ring2/occ4", score32b_kernel<2, 4, true, 1>, 1},
+    {"bcast/ring4/occ4", score32b_kernel<4, 4, false, 1>, 1},
+    {"bcast/ring2/occ5", score32b_kernel<2, 5, false, 1>, 1},
+    {"bcast/ring4/occ4/wg4", score32b_kernel<4, 4, false, 4>, 4},
+    {"readlane/unroll4/occ4", score32r_kernel<4, 4>, kWaves},
+    {"readlane/unroll4/occ3", score32r_kernel<4, 3>, kWaves},
+    {"mfma/unroll4/occ3", score32m_kernel<4, 3>, kWaves},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 }  // namespace
@@ -967,12 +1065,13 @@ hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     // are cut into chunks (the kernels address partials by glowworm and
     // block, so a chunk only needs the offset work pointer).
     constexpr int kChunk = (1 << 26) - kWaves;
+    const int wpb = kVariants[variant].waves;
     for (int base = 0; base < a.n_work; base += kChunk) {
         ScoreArgs part = a;
         part.work = a.work + 2 * (size_t)base;
         part.n_work = min(kChunk, a.n_work - base);
-        const int grid = (part.n_work + kWaves - 1) / kWaves;
-        hipLaunchKernelGGL(kVariants[variant].kernel, dim3(grid), dim3(kScore32Threads), 0, s,
+        const int grid = (part.n_work + wpb - 1) / wpb;
+        hipLaunchKernelGGL(kVariants[variant].kernel, dim3(grid), dim3(64 * wpb), 0, s,
                            part);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
