@@ -161,11 +161,12 @@ struct sdgso_complex_s {
     DevBuf<float4> tiles, lblocks;
     DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
-    DevBuf<float4> rpos32;     // [ntiles*64] f32 coordinates - tile centre
+    DevBuf<float4> rpos32;     // [ntiles*64] f32 coordinates (near test)
     DevBuf<double> rrel4;      // [ntiles*64][4] f64 coordinates - tile centre
     DevBuf<float> rmodA;       // [ntiles*64][rmod_stride] modes by slot
     int rmod_stride = 0;
     DevBuf<int> rtype_s, rperm, lperm;
+    DevBuf<double> lblk;       // [nlblocks][3 + 3*nml][64] ligand coordinates and modes by block slot
     DevComplex dev{};          // descriptor with device pointers
     double max_rec_extent = 0; // |xyz| maxima, for the guard-band bound
 };
@@ -238,7 +239,8 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
     std::vector<float> m32((size_t)nm * 3 * slots, 0.0f);
     std::vector<int> ts(slots, 0), perm(slots, -1);
     // whole groups of four modes (twelve floats) per slot
-    const int stride = (3 * nm + 11) / 12 * 12;
+    // modes of a slot: 3*nm floats (mode-major x, y, z) in mode_chunks(nm) float4
+    const int stride = 4 * mode_chunks(nm);
     std::vector<float4> p32(slots);
     std::vector<double> r4(4 * slots, 0.0);
     std::vector<float> mA((size_t)stride * slots, 0.0f);
@@ -280,8 +282,9 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
                 for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = 1.0e4;
             }
             for (int k = 0; k < 3; k++) r4[4 * (size_t)s + k] = rrel[(size_t)k * slots + s];
-            p32[s] = make_float4((float)r4[4 * (size_t)s], (float)r4[4 * (size_t)s + 1],
-                                 (float)r4[4 * (size_t)s + 2], 0.0f);
+            p32[s] = make_float4((float)(r4[4 * (size_t)s] + (double)cf[0]),
+                                 (float)(r4[4 * (size_t)s + 1] + (double)cf[1]),
+                                 (float)(r4[4 * (size_t)s + 2] + (double)cf[2]), 0.0f);
         }
     }
     for (int m = 0; m < nm; m++) {
@@ -349,8 +352,21 @@ static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<
         }
         c->dev.lmode_max[m] = (float)(mx * (1 + 1e-5));
     }
+    // coordinates and modes of every block slot, slot-contiguous (padding: zeros)
+    const int rows = 3 + 3 * nm;
+    std::vector<double> lb((size_t)nb * rows * kBlock, 0.0);
+    for (int s = 0; s < nb * kBlock; s++) {
+        const int i = perm[s];
+        if (i < 0) continue;
+        double *B = lb.data() + (size_t)(s / kBlock) * rows * kBlock + s % kBlock;
+        for (int k = 0; k < 3; k++) B[(size_t)k * kBlock] = xyz[3 * (size_t)i + k];
+        for (int m = 0; m < nm; m++)
+            for (int k = 0; k < 3; k++)
+                B[(size_t)(3 + 3 * m + k) * kBlock] = msoa[((size_t)m * 3 + k) * n + i];
+    }
     c->lblocks.upload(blocks);
     c->lperm.upload(perm);
+    c->lblk.upload(lb);
     c->dev.nlblocks = nb;
 }
 
@@ -437,6 +453,7 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.rperm = c->rperm.p;
         d.lperm = c->lperm.p;
         d.lblocks = c->lblocks.p;
+        d.lblk = c->lblk.p;
         d.rrel = c->rrel.p;
         *out = c.release();
     });
@@ -482,6 +499,7 @@ static ScoreArgs score_args(sdgso_run r) {
     a.partial = r->partial.p;
     a.partial_pairs = r->partial_pairs.p;
     a.max_lblocks = r->max_lblocks;
+    a.nmr = r->nmr;
     a.work = r->work.p;
     a.n_work = (int)r->work.n;
     return a;

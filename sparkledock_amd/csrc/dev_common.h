@@ -116,6 +116,48 @@ __device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double 
     z += pose[2];
 }
 
+// The same for slot `lane` of ligand block b, read from the block-slot copy
+// (every load of a wave is contiguous); the arithmetic is ligand_atom64's.
+__device__ __forceinline__ void ligand_slot64(const DevComplex &c, const double *pose,
+                                              const double inv[4], int b, int lane, double &x,
+                                              double &y, double &z) {
+    const int nm = c.nml;
+    const double *B = c.lblk + (size_t)b * (3 + 3 * nm) * kBlock + lane;
+    x = gld(B, 0);
+    y = gld(B, kBlock);
+    z = gld(B, 2 * kBlock);
+    const double *ext = pose + 7 + c.nmr;
+    int m = 0;
+    for (; m + 4 <= nm; m += 4) {
+        double mx[4], my[4], mz[4], e[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const double *M = B + (size_t)(3 + 3 * (m + k)) * kBlock;
+            mx[k] = gld(M, 0);
+            my[k] = gld(M, kBlock);
+            mz[k] = gld(M, 2 * kBlock);
+            e[k] = ext[m + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            x += mx[k] * e[k];
+            y += my[k] * e[k];
+            z += mz[k] * e[k];
+        }
+    }
+    for (; m < nm; m++) {
+        const double *M = B + (size_t)(3 + 3 * m) * kBlock;
+        const double e = ext[m];
+        x += gld(M, 0) * e;
+        y += gld(M, kBlock) * e;
+        z += gld(M, 2 * kBlock) * e;
+    }
+    rotate_point(pose + 3, inv, x, y, z);
+    x += pose[0];
+    y += pose[1];
+    z += pose[2];
+}
+
 // Ligand reference point of a pose (rotation + translation, no modes).
 __device__ __forceinline__ void ref_point64(const DevComplex &c, const double *pose,
                                             const double inv[4], double out[3]) {

@@ -191,7 +191,7 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     x.L = make_int4(-kFar, -kFar, -kFar, 0);
     if (x.lig >= 0) {
         double px, py, pz;
-        ligand_atom64(c, x.pose, inv, x.lig, px, py, pz);
+        ligand_slot64(c, x.pose, inv, b, x.lane, px, py, pz);
         x.L = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
                         gld(c.ltype, x.lig));
     }
@@ -631,7 +631,39 @@ __device__ __forceinline__ void static_for(F &&f) {
     }
 }
 
-// A pointer the whole wave holds (scalar registers).
+// v from the lane the DPP control kCtrl selects (0 where it selects none).
+template <int kCtrl, int kRowMask = 0xF>
+__device__ __forceinline__ float dpp_f32(float v) {
+    return __builtin_bit_cast(
+        float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, kRowMask, 0xF, false));
+}
+
+// Sum of v over aligned groups of kC lanes (8, 16 or 32), exact in the
+// group's last lane: quad swaps, then row_shr:4 (8-lane sums in lanes 4-7 and
+// 12-15 of a row), row_shr:8 (16-lane sums in lanes 12-15) and row_bcast:15
+// into rows 1 and 3 (32-lane sums in lanes 16-31 of each half).
+template <int kC>
+__device__ __forceinline__ float group_sum(float v) {
+    static_assert(kC == 8 || kC == 16 || kC == 32, "group of 8, 16 or 32 lanes");
+    v += dpp_f32<0xB1>(v);  // quad_perm [1, 0, 3, 2]
+    v += dpp_f32<0x4E>(v);  // quad_perm [2, 3, 0, 1]
+    v += dpp_f32<0x114>(v);
+    if constexpr (kC >= 16) v += dpp_f32<0x118>(v);
+    if constexpr (kC >= 32) v += dpp_f32<0x142, 0xA>(v);
+    return v;
+}
+
+// Values the whole wave holds (scalar registers).
+__device__ __forceinline__ float uniform_f32(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_readfirstlane(__builtin_bit_cast(int, v)));
+}
+__device__ __forceinline__ double uniform_f64(double v) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
 template <class T>
 __device__ __forceinline__ T *uniform_ptr(T *p) {
     const unsigned long long v = (unsigned long long)p;
@@ -640,14 +672,15 @@ __device__ __forceinline__ T *uniform_ptr(T *p) {
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
-template <int kUnroll, int kMinWaves, bool kMfma, int kWpb>
+template <int kUnroll, int kMinWaves, bool kMfma, int kWpb, int kTilesB, int kC>
 __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArgs a) {
+    constexpr int kPend = 64 * kTilesB + 64;  // pending near slots: a batch's worth + one round
     constexpr int kQueue = queue_len<kUnroll, kMfma>();
     constexpr int kGroup = 64 * kUnroll;  // pairs of one group of rounds
     __shared__ int4 s_rt[kWpb][kRtRows];     // packed near receptor atoms: fixed point + row
     __shared__ float4 s_rf[kWpb][64];        //   this batch's atoms in A: x, y, z, |r|^2
     __shared__ unsigned short s_rs[kWpb][2 * 64];  // receptor slots of the packed atoms
-    __shared__ unsigned short s_pd[kWpb][2 * 64];  // near slots waiting for displacement
+    __shared__ unsigned short s_pd[kWpb][kPend];   // near slots waiting for displacement
     __shared__ int4 s_lg[kWpb][64];          // the block's ligand atoms: fixed point + type*20
     __shared__ int s_q[kWpb][kQueue];        // queued pairs (LDS offsets of both atoms)
     __shared__ float s_ef[kWpb][SDGSO_MAX_MODES];
@@ -661,7 +694,6 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // the fields the tile visits and the evaluation read, in scalar registers
     const int ntiles = __builtin_amdgcn_readfirstlane(c.ntiles);
     const int nmr = __builtin_amdgcn_readfirstlane(c.nmr);
-    const int mstride4 = __builtin_amdgcn_readfirstlane(c.rmod_stride / 4);
     const float4 *tiles = uniform_ptr(c.tiles);
     const float4 *rpos32 = uniform_ptr(c.rpos32);
     const double *rrel4 = uniform_ptr(c.rrel4);
@@ -681,6 +713,10 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     wave_setup(a, c, gid, b, ef, ct, x, kCandB);
     const int lane = x.lane;
     lg[lane] = x.L;
+    // the set-up's wave-uniform results, in scalar registers
+    const double cb0 = uniform_f64(x.cb[0]), cb1 = uniform_f64(x.cb[1]), cb2 = uniform_f64(x.cb[2]);
+    const float fc0 = uniform_f32(x.fc[0]), fc1 = uniform_f32(x.fc[1]), fc2 = uniform_f32(x.fc[2]);
+    const float reach = uniform_f32(x.reach);
     if (lane == 0) rt[2 * 64] = make_int4(kFar, kFar, kFar, 0);
     wave_sync_lds();
     // queue words: LDS byte offsets from s_rt[0][0] (high half) and s_lg[0][0] (low half)
@@ -690,17 +726,17 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     const unsigned lane_lo = (unsigned)(w * 64 + lane) * 16u;
     const int null_tag = (int)(((rt0 + 2 * 64 * 16u) << 16) | (unsigned)(w * 64) * 16u);
     const float bandq = a.band * kQ2;
-    const float thr = 225.0f + 2.0f * a.band + 0x1p-18f * x.reach * x.reach + 1e-3f;
-    const float reach2 = x.reach * x.reach;
+    const float thr = 225.0f + 2.0f * a.band + 0x1p-18f * reach * reach + 1e-3f;
+    const float reach2 = reach * reach;
     // this lane's pre-test operands: -2 l and T - |l|^2 (padding lanes never pass)
     const float lx = (float)x.L.x * kInvQ, ly = (float)x.L.y * kInvQ, lz = (float)x.L.z * kInvQ;
     const float m2x = -2.f * lx, m2y = -2.f * ly, m2z = -2.f * lz;
     const float tl = x.lig >= 0 ? thr - __builtin_fmaf(lz, lz, __builtin_fmaf(ly, ly, lx * lx))
                                 : -1e30f;
     // MFMA pre-test: B operands of the two 32-atom column groups, and the threshold
-    const bool mfma_ok = kMfma && x.rmax < 250.0f;
-    const float thr16 = 225.0f + 2.0f * a.band + 0x1p-8f * x.reach * (x.reach - 15.0f) +
-                        0x1p-17f * x.reach * x.reach + 1e-3f;
+    const bool mfma_ok = kMfma && uniform_f32(x.rmax) < 250.0f;
+    const float thr16 = 225.0f + 2.0f * a.band + 0x1p-8f * reach * (reach - 15.0f) +
+                        0x1p-17f * reach * reach + 1e-3f;
     half4 bop[2];
     if (kMfma) {
 #pragma unroll
@@ -804,41 +840,74 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // queued at the end are evaluated too, since the next batch overwrites it.
     auto batch = [&](int n) {
         old = qn;
+        // this lane's atom: slot, tile centre, coordinates and type row (loads first)
+        int slot = 0, row = 0;
+        float4 tc = make_float4(0.f, 0.f, 0.f, 0.f);
+        double2 rxy = make_double2(0.0, 0.0);
+        double rz = 0.0;
         if (lane < n) {
-            const int slot = pd[lane];
-            const float4 tc = gld(tiles, slot / kTile);
-            const double2 rxy = gld2(rrel4, 2 * (size_t)slot);
-            const double rz = gld(rrel4, 4 * (size_t)slot + 2);
-            float Dx = 0.f, Dy = 0.f, Dz = 0.f;
-            const float4 *MA = rmodA + (size_t)slot * mstride4;
-            const int ngroups = (nmr + 3) / 4;  // groups of four modes: twelve floats, three loads
-            for (int g0 = 0; g0 < ngroups; g0 += 3) {
-                // up to three groups per round, loads issued together (a group past
-                // the last one re-reads the last; its products are not added)
-                float4 q[9];
+            slot = pd[lane];
+            tc = gld(tiles, slot / kTile);
+            rxy = gld2(rrel4, 2 * (size_t)slot);
+            rz = gld(rrel4, 4 * (size_t)slot + 2);
+            row = gld(rtype_s, slot);
+        }
+        // displacement D = sum_m M_m e_m of the batch's atoms, kC lanes per atom: in
+        // round k lane l takes float4 j = l % kC of atom k * 64 / kC + l / kC (an
+        // atom's modes are kC float4 in whole 128-byte lines, so a round reads
+        // 64 / kC lines), weighs its four floats by their modes' extents per
+        // component, and the group's sums reach its last lane; they are staged
+        // in rf, which the packing below overwrites
+        float4 D = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (nmr > 0) {
+            const int j = lane & (kC - 1);
+            float wx[4], wy[4], wz[4];
 #pragma unroll
-                for (int j = 0; j < 3; j++)
-                    for (int h = 0; h < 3; h++) q[3 * j + h] = gld(MA, 3 * min(g0 + j, ngroups - 1) + h);
+            for (int t = 0; t < 4; t++) {
+                const int f = 4 * j + t, m = f / 3, comp = f - 3 * m;
+                const float e = f < 3 * nmr ? ef[min(m, SDGSO_MAX_MODES - 1)] : 0.f;
+                wx[t] = comp == 0 ? e : 0.f;
+                wy[t] = comp == 1 ? e : 0.f;
+                wz[t] = comp == 2 ? e : 0.f;
+            }
+            constexpr int kApr = 64 / kC;  // atoms per round
+            const int nround = (n + kApr - 1) / kApr;
+            for (int k0 = 0; k0 < nround; k0 += 4) {
+                float4 q[4];
+                int av[4];
 #pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    const float mv[12] = {q[3 * j].x,     q[3 * j].y,     q[3 * j].z,     q[3 * j].w,
-                                          q[3 * j + 1].x, q[3 * j + 1].y, q[3 * j + 1].z, q[3 * j + 1].w,
-                                          q[3 * j + 2].x, q[3 * j + 2].y, q[3 * j + 2].z, q[3 * j + 2].w};
+                for (int r = 0; r < 4; r++) {
+                    av[r] = (k0 + r) * kApr + lane / kC;
+                    q[r] = (k0 + r < nround && av[r] < n)
+                               ? gld(rmodA, (size_t)pd[av[r]] * kC + j)
+                               : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
 #pragma unroll
-                    for (int k = 0; k < 4; k++) {
-                        const int m = 4 * (g0 + j) + k;
-                        if (m < nmr) {
-                            Dx = __builtin_fmaf(mv[3 * k], ef[m], Dx);
-                            Dy = __builtin_fmaf(mv[3 * k + 1], ef[m], Dy);
-                            Dz = __builtin_fmaf(mv[3 * k + 2], ef[m], Dz);
-                        }
-                    }
+                for (int r = 0; r < 4; r++) {
+                    if (k0 + r >= nround) break;
+                    float px = q[r].x * wx[0], py = q[r].x * wy[0], pz = q[r].x * wz[0];
+                    px = __builtin_fmaf(q[r].y, wx[1], px);
+                    py = __builtin_fmaf(q[r].y, wy[1], py);
+                    pz = __builtin_fmaf(q[r].y, wz[1], pz);
+                    px = __builtin_fmaf(q[r].z, wx[2], px);
+                    py = __builtin_fmaf(q[r].z, wy[2], py);
+                    pz = __builtin_fmaf(q[r].z, wz[2], pz);
+                    px = __builtin_fmaf(q[r].w, wx[3], px);
+                    py = __builtin_fmaf(q[r].w, wy[3], py);
+                    pz = __builtin_fmaf(q[r].w, wz[3], pz);
+                    px = group_sum<kC>(px);
+                    py = group_sum<kC>(py);
+                    pz = group_sum<kC>(pz);
+                    if (j == kC - 1 && av[r] < n) rf[av[r]] = make_float4(px, py, pz, 0.f);
                 }
             }
-            const int4 R = make_int4(qround(((double)tc.x - x.cb[0]) + rxy.x + (double)Dx),
-                                     qround(((double)tc.y - x.cb[1]) + rxy.y + (double)Dy),
-                                     qround(((double)tc.z - x.cb[2]) + rz + (double)Dz),
-                                     gld(rtype_s, slot));
+            wave_sync_lds();
+            if (lane < n) D = rf[lane];
+        }
+        if (lane < n) {
+            const int4 R = make_int4(qround(((double)tc.x - cb0) + rxy.x + (double)D.x),
+                                     qround(((double)tc.y - cb1) + rxy.y + (double)D.y),
+                                     qround(((double)tc.z - cb2) + rz + (double)D.z), row);
             const float fx = (float)R.x * kInvQ, fy = (float)R.y * kInvQ, fz = (float)R.z * kInvQ;
             rt[64 * p + lane] = R;
             rf[lane] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
@@ -930,11 +999,11 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // against the block sphere; near slots join the pending list.
     int npd = 0;
     long long n_cand = 0;
-    // near test of this lane's atom of tile t (centre tc, atom pp); near slots are appended
-    auto visit = [&](int t, float4 tc, float4 pp) {
-        const float ax = pp.x + (tc.x - x.fc[0]);
-        const float ay = pp.y + (tc.y - x.fc[1]);
-        const float az = pp.z + (tc.z - x.fc[2]);
+    // near test of this lane's atom of tile t (position pp); near slots are appended
+    auto visit = [&](int t, float4 pp) {
+        const float ax = pp.x - fc0;
+        const float ay = pp.y - fc1;
+        const float az = pp.z - fc2;
         const bool nearb = ax * ax + ay * ay + az * az <= reach2;
         const unsigned long long m = __ballot(nearb);
         if (!m) return;
@@ -954,32 +1023,32 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             k = min(k, nc - 1);
             return k < 64 ? __builtin_amdgcn_readlane(c0, k) : __builtin_amdgcn_readlane(c1, k - 64);
         };
-        for (int k = 0; k < nc;) {
-            // two tiles in flight: A is visited while B loads, then the roles swap;
-            // a full pending list leaves the loop for the batch (one call site)
-            int tA = tile_of(k);
-            float4 tcA = gld(tiles, tA), pA = gld(rpos32, tA * kTile + lane);
-            for (;;) {
-                const int tB = tile_of(k + 1);
-                const float4 tcB = gld(tiles, tB), pB = gld(rpos32, tB * kTile + lane);
-                visit(tA, tcA, pA);
-                k++;
-                if (npd >= 64 || k >= nc) break;
-                tA = tile_of(k + 1);
-                tcA = gld(tiles, tA);
-                pA = gld(rpos32, tA * kTile + lane);
-                visit(tB, tcB, pB);
-                k++;
-                if (npd >= 64 || k >= nc) break;
+        for (int k = 0; k < nc; k += kTilesB) {
+            // kTilesB tiles per round, their loads in flight together (a tile past
+            // the list's end re-reads the last one and is not visited)
+            int tt[kTilesB];
+            float4 pp[kTilesB];
+#pragma unroll
+            for (int j = 0; j < kTilesB; j++) {
+                tt[j] = tile_of(k + j);
+                pp[j] = gld(rpos32, tt[j] * kTile + lane);
             }
-            if (npd >= 64) {
+#pragma unroll
+            for (int j = 0; j < kTilesB; j++)
+                if (k + j < nc) visit(tt[j], pp[j]);
+            while (npd >= 64) {
                 wave_sync_lds();
                 batch(64);
                 // move the remaining pending slots to the front
                 const int rem = npd - 64;
-                const unsigned short v = lane < rem ? pd[64 + lane] : (unsigned short)0;
+                unsigned short v[kTilesB];
+#pragma unroll
+                for (int j = 0; j < kTilesB; j++)
+                    v[j] = 64 * j + lane < rem ? pd[64 + 64 * j + lane] : (unsigned short)0;
                 wave_sync_lds();
-                if (lane < rem) pd[lane] = v;
+#pragma unroll
+                for (int j = 0; j < kTilesB; j++)
+                    if (64 * j + lane < rem) pd[64 * j + lane] = v[j];
                 npd = rem;
             }
         }
@@ -1036,18 +1105,23 @@ __global__ void advance_kernel(int *k) {
 namespace {
 struct Variant {
     const char *name;
-    void (*kernel)(ScoreArgs);
-    int waves;  // waves per workgroup
+    void (*kernel[3])(ScoreArgs);  // for 8, 16 and 32 float4 of modes per receptor slot
+    int waves;                     // waves per workgroup
 };
+#define SD_BCAST(R, O, M, T) \
+    { score32b_kernel<R, O, M, 1, T, 8>, score32b_kernel<R, O, M, 1, T, 16>, score32b_kernel<R, O, M, 1, T, 32> }
+#define SD_SAME(K) \
+    { K, K, K }
 const Variant kVariants[] = {
-    {"bcast-mfma16/ring3/occ4", score32b_kernel<3, 4, true, 1>, 1},
-    {"bcast-mfma16/ring2/occ4", score32b_kernel<2, 4, true, 1>, 1},
-    {"bcast/ring4/occ4", score32b_kernel<4, 4, false, 1>, 1},
-    {"bcast/ring2/occ5", score32b_kernel<2, 5, false, 1>, 1},
-    {"bcast/ring4/occ4/wg4", score32b_kernel<4, 4, false, 4>, 4},
-    {"readlane/unroll4/occ4", score32r_kernel<4, 4>, kWaves},
-    {"readlane/unroll4/occ3", score32r_kernel<4, 3>, kWaves},
-    {"mfma/unroll4/occ3", score32m_kernel<4, 3>, kWaves},
+    {"bcast-mfma16/ring3/occ4", SD_BCAST(3, 4, true, 4), 1},
+    {"bcast-mfma16/ring2/occ4", SD_BCAST(2, 4, true, 4), 1},
+    {"bcast/ring4/occ4", SD_BCAST(4, 4, false, 4), 1},
+    {"bcast/ring2/occ5", SD_BCAST(2, 5, false, 4), 1},
+    {"bcast/ring4/occ5", SD_BCAST(4, 5, false, 4), 1},
+    {"bcast/ring4/occ6", SD_BCAST(4, 6, false, 4), 1},
+    {"readlane/unroll4/occ4", SD_SAME((score32r_kernel<4, 4>)), kWaves},
+    {"readlane/unroll4/occ3", SD_SAME((score32r_kernel<4, 3>)), kWaves},
+    {"mfma/unroll4/occ3", SD_SAME((score32m_kernel<4, 3>)), kWaves},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 }  // namespace
@@ -1066,12 +1140,14 @@ hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     // block, so a chunk only needs the offset work pointer).
     constexpr int kChunk = (1 << 26) - kWaves;
     const int wpb = kVariants[variant].waves;
+    const int kc = mode_chunks(a.nmr);
+    void (*kernel)(ScoreArgs) = kVariants[variant].kernel[kc == 8 ? 0 : (kc == 16 ? 1 : 2)];
     for (int base = 0; base < a.n_work; base += kChunk) {
         ScoreArgs part = a;
         part.work = a.work + 2 * (size_t)base;
         part.n_work = min(kChunk, a.n_work - base);
         const int grid = (part.n_work + wpb - 1) / wpb;
-        hipLaunchKernelGGL(kVariants[variant].kernel, dim3(grid), dim3(64 * wpb), 0, s,
+        hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * wpb), 0, s,
                            part);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

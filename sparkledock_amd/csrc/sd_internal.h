@@ -18,6 +18,7 @@
 //   ligand blocks (FP32 guard form): 64 atoms per block, grouped by bisection
 //     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
 //     lblocks f32x4 [nlblocks]     block centre in the ligand frame, radius
+//     lblk    f64 [nlblocks][3+3*nml][64]  coordinates and modes of the block slots
 //   run state, N = sum(swarms) * G glowworms
 //     pos [N][P] f64, luciferin/score/vision [N] f64, ref [N][3] f64,
 //     nnei/moved/select/step [N] i32, pairs [N] i64,
@@ -41,6 +42,13 @@ constexpr double kCutoff2 = 225.0;
 constexpr int kMaxTiles32 = 1024;      // FP32 form: receptor tiles per complex (65536 atoms)
 constexpr int kStatSlots = 256;        // slots per filter-statistics counter
 
+// float4 per receptor slot that hold its FP32 modes (rmodA): 3*nm floats in
+// whole 128-byte lines, rounded up to 8, 16 or 32 float4 (nm <= 32).
+inline __host__ __device__ int mode_chunks(int nm) {
+    const int lines = (3 * nm + 31) / 32;
+    return lines <= 1 ? 8 : (lines <= 2 ? 16 : 32);
+}
+
 // Per-complex device descriptor (plain data; lives in a device array).
 struct DevComplex {
     int nr, nl, nmr, nml;
@@ -58,8 +66,8 @@ struct DevComplex {
     const double *rrel;      // [3][ntiles*64] coordinates - tile centre (f64)
     const float *rmod32;     // [nmr][3][ntiles*64] modes (f32)
     // the same per slot, for gathers by slot: near-test coordinates (f32,
-    // w = 0), coordinates (f64 x, y, z, 0) and modes (f32, mode-major x, y, z,
-    // padded to a multiple of 4 floats: rmod_stride)
+    // absolute, w = 0), coordinates - tile centre (f64 x, y, z, 0) and modes
+    // (f32, mode-major x, y, z, padded to 4 * mode_chunks(nmr) = rmod_stride floats)
     const float4 *rpos32;
     const double *rrel4;
     const float *rmodA;
@@ -71,6 +79,7 @@ struct DevComplex {
     int nlblocks;
     const int *lperm;
     const float4 *lblocks;
+    const double *lblk;      // [nlblocks][3 + 3*nml][64] coordinates, then modes, by block slot
     float lmode_max[SDGSO_MAX_MODES];
 };
 
@@ -94,6 +103,7 @@ struct ScoreArgs {
     double *partial;        // [n_glow][max_lblocks] per-block energies (FP32 form)
     long long *partial_pairs;  // [n_glow][max_lblocks] per-block pair counts
     int max_lblocks;
+    int nmr;                // receptor modes of every complex of the launch
     const int *work;        // FP32 form: work list of (glowworm, ligand block), or null
     int n_work;
     // FP32 form, optional: filter statistics [6][kStatSlots] (waves, waves
