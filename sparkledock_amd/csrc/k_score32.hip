@@ -29,10 +29,13 @@
 // (integer differences are exact and convert to FP32 exactly below 16 A), the
 // bin, and the table value; several gathers are in flight per lane before the
 // compensated (Kahan) FP32 adds.  The FP32 d2 differs from the FP64 reference
-// d2 by at most 9.5e-5 A^2, under the default 1e-4 A^2 band; a pair within the
-// band of a boundary k^2/4 (k = 0..30: d2 = 0, every bin edge and the 225 A^2
-// cutoff) is recomputed in FP64 from the reference-order coordinates, so every
-// bin and in/out decision equals the reference's.  Outside every band a pair
+// d2 by at most 9.04e-5 + 3.1e-6 k S A^2 (S the pose's receptor displacement
+// bound, k the FP32 roundings per mode term): 9.7e-5 A^2 for the synthetic
+// workloads, under the default 1e-4 A^2 band, which a wave widens to the bound
+// when the bound is larger.  A pair within the band of a boundary k^2/4
+// (k = 0..30: d2 = 0, every bin edge and the 225 A^2 cutoff) is recomputed in
+// FP64 from the reference-order coordinates, so every bin and in/out decision
+// equals the reference's.  Outside every band a pair
 // is inside iff d2 < 225 and its bin is floor(2 sqrt(d2)) through dist_to_bins
 // (raw v_sqrt_f32, 1 ulp, cannot cross an integer there).
 //
@@ -130,6 +133,7 @@ struct WaveCtx {
     double cb[3];     // block centre (fixed-point origin)
     float fc[3];
     float reach;      // cutoff + block radius + displacement bounds
+    float band;       // this wave's guard band (A^2): the run's, widened to the error bound
     float rmax;       // bound of |r| for the receptor atoms that pass the atom test
     int4 L;           // this lane's ligand atom: fixed point + type*20
     int lig;          // its original index (-1 = padding)
@@ -202,6 +206,15 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     for (int m = 0; m < c.nml; m++) dl += fabsf((float)lext[m]) * c.lmode_max[m];
     dr = dr * 1.0001f + 1e-3f;
     dl = dl * 1.0001f + 1e-3f;
+    // Guard band that keeps this wave's bins exact.  Per axis the fixed-point
+    // coordinates differ from the reference's FP64 ones by at most 1.01 grid
+    // units (two roundings to 2^-20 A, FP64 noise) plus the FP32 displacement
+    // error k u S, u = 2^-24, S <= dr the displacement bound and k the
+    // roundings per mode term (FP32 mode and extent, and the sum: at most 11 for
+    // the broadcast variant's in-lane FMAs and group sum, nmr + 2 for a chain
+    // in mode order).  With |dx| + |dy| + |dz| <= 15 sqrt(3) and 3 u 225 for
+    // the FP32 sum of squares, |d2 - d2_ref| <= 9.04e-5 + 3.098e-6 k S A^2.
+    x.band = fmaxf(a.band, 9.1e-5f + 3.11e-6f * (float)max(11, c.nmr + 2) * dr);
     x.reach = 15.0f + blk.w + dl + dr + 0.01f;
     x.rmax = x.reach + dr + 0.01f;
     x.fc[0] = (float)x.cb[0];
@@ -309,8 +322,8 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
     WaveCtx x;
     wave_setup(a, c, gid, b, ef, ct, x);
     const int lane = x.lane;
-    const float bandq = a.band * kQ2;
-    const float cutq = (225.0f + 2.0f * a.band) * kQ2;
+    const float bandq = x.band * kQ2;
+    const float cutq = (225.0f + 2.0f * x.band) * kQ2;
     const int slots = c.ntiles * kTile;
 
     Kahan acc;
@@ -436,8 +449,8 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
     wave_setup(a, c, gid, b, ef, ct, x);
     const int lane = x.lane;
     lg[lane] = x.L;
-    const float bandq = a.band * kQ2;
-    const float thr = 225.0f + 2.0f * a.band + kFilterMargin;
+    const float bandq = x.band * kQ2;
+    const float thr = 225.0f + 2.0f * x.band + kFilterMargin;
     const int slots = c.ntiles * kTile;
     const bool lo_half = lane < 32;
     wave_sync_lds();
@@ -725,8 +738,9 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     const unsigned rt0 = (unsigned)(w * kRtRows) * 16u;
     const unsigned lane_lo = (unsigned)(w * 64 + lane) * 16u;
     const int null_tag = (int)(((rt0 + 2 * 64 * 16u) << 16) | (unsigned)(w * 64) * 16u);
-    const float bandq = a.band * kQ2;
-    const float thr = 225.0f + 2.0f * a.band + 0x1p-18f * reach * reach + 1e-3f;
+    const float band = uniform_f32(x.band);
+    const float bandq = band * kQ2;
+    const float thr = 225.0f + 2.0f * band + 0x1p-18f * reach * reach + 1e-3f;
     const float reach2 = reach * reach;
     // this lane's pre-test operands: -2 l and T - |l|^2 (padding lanes never pass)
     const float lx = (float)x.L.x * kInvQ, ly = (float)x.L.y * kInvQ, lz = (float)x.L.z * kInvQ;
@@ -735,7 +749,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                                 : -1e30f;
     // MFMA pre-test: B operands of the two 32-atom column groups, and the threshold
     const bool mfma_ok = kMfma && uniform_f32(x.rmax) < 250.0f;
-    const float thr16 = 225.0f + 2.0f * a.band + 0x1p-8f * reach * (reach - 15.0f) +
+    const float thr16 = 225.0f + 2.0f * band + 0x1p-8f * reach * (reach - 15.0f) +
                         0x1p-17f * reach * reach + 1e-3f;
     half4 bop[2];
     if (kMfma) {
