@@ -1132,7 +1132,6 @@ const Variant kVariants[] = {
     {"bcast/ring4/occ4", SD_BCAST(4, 4, false, 4), 1},
     {"bcast/ring2/occ5", SD_BCAST(2, 5, false, 4), 1},
     {"bcast/ring4/occ5", SD_BCAST(4, 5, false, 4), 1},
-    {"bcast/ring4/occ6", SD_BCAST(4, 6, false, 4), 1},
     {"readlane/unroll4/occ4", SD_SAME((score32r_kernel<4, 4>)), kWaves},
     {"readlane/unroll4/occ3", SD_SAME((score32r_kernel<4, 3>)), kWaves},
     {"mfma/unroll4/occ3", SD_SAME((score32m_kernel<4, 3>)), kWaves},
