@@ -51,8 +51,12 @@ def test_score_fp64_bit_exact(seed, nmr, nml):
     assert np.array_equal(s, s0)
 
 
-@pytest.mark.parametrize("seed,nmr,nml", [(4, 10, 10), (5, 0, 0), (6, 20, 20)])
+@pytest.mark.parametrize("seed,nmr,nml", [(4, 10, 10), (5, 0, 0), (6, 20, 20), (9, 24, 6), (10, 32, 1),
+                                          (11, 1, 0)])
 def test_score_fp32_guard(seed, nmr, nml):
+    """Receptor mode counts cover every mode-line width of the broadcast
+    variant's cooperative displacement: 8 float4 (up to 10 modes), 16 (up to
+    21) and 32 (up to 32), and none."""
     c = GC.case(seed, 700, 300, nmr, nml, 4, 16)
     pr = GC.Pair(c)
     poses = poses_of(c)
