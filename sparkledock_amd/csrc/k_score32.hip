@@ -593,8 +593,9 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 
 // ---------------------------------------------------------------- broadcast variant
 // Near receptor atoms are collected across the visited tiles (their slots in a
-// pending list) and displaced 64 at a time, one per lane, so every lane of
-// the displacement has an atom; the displaced atoms are packed into LDS and
+// pending list) and displaced 64 at a time: eight lanes read an atom's mode
+// line together and sum their products across lanes, then each lane rounds
+// one atom to the block's grid.  The displaced atoms are packed into LDS and
 // the wave walks them with broadcast LDS reads, four per round.  Each lane
 // pre-tests its ligand atom with the FP32 form W = |r|^2 - 2 r.l against
 // T - |l|^2 (A^2), and the pairs that pass are queued as one word: the LDS
