@@ -714,7 +714,7 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
             }
             r->work.upload(work);
             r->work.n = work.size() / 2;
-            r->variants.assign(n_complex, 0);
+            r->variants.assign(n_complex, score32_default_variant());
         }
         HIPCHK(hipStreamCreateWithFlags(&r->own_stream, hipStreamNonBlocking));
         r->stream_h = r->own_stream;

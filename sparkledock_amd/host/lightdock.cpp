@@ -366,6 +366,13 @@ int main(int argc, char **argv) {
     sdgso_run run = nullptr;
     check(sdgso_run_create(1, &cx, &S, G, pos.data(), u.data(), steps, &prm, form, 1e-4, &run),
           "sdgso_run_create");
+    if (form == SDGSO_FP32_GUARD) {
+        // pick the FP32 scoring kernel for this complex (scores the starting
+        // poses a few times; the run's state is unchanged)
+        int chosen = 0;
+        check(sdgso_run_tune(run, 1, nullptr, &chosen), "sdgso_run_tune");
+        std::printf("scoring kernel: %s\n", sdgso_tune_variant_name(chosen));
+    }
     std::printf("prepare docking\n");
     const auto t0 = std::chrono::steady_clock::now();
     check(sdgso_run_steps(run, steps, 1), "sdgso_run_steps");
