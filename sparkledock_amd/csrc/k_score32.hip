@@ -824,6 +824,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     auto flush_groups = [&]() {
         const int ng = qn / kGroup;
         if (ng == 0) return;
+        n_queued += kGroup * ng;
         wave_sync_lds();
         eval_groups(ng);
         const int rem = qn - kGroup * ng;
@@ -841,6 +842,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // Evaluate everything (after flush_groups: qn < kGroup), padding the last round.
     auto flush_all = [&]() {
         if (qn == 0) return;
+        n_queued += qn;
         const int pad = (-qn) & 63;
         if (lane < pad) qq[qn + lane] = null_tag;
         wave_sync_lds();
@@ -971,7 +973,6 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
                                     (int)(tagb + ((16u * (unsigned)(8 * (v >> 2) + (v & 3))) << 16));
                             qn += __popcll(m);
-                            n_queued += __popcll(m);
                         }
                     }
                 }
@@ -1000,7 +1001,6 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                                         __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
                                 (int)(((hib + 16u * (unsigned)(i0 + j)) << 16) | lane_lo);
                         qn += __popcll(m[j]);
-                        n_queued += __popcll(m[j]);
                     }
                 }
                 if (qn > kQueue - 4 * 64) flush_groups();
