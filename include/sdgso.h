@@ -140,8 +140,10 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
 /* Autotuner of the FP32 guard-band scoring kernel (no counterpart in the
  * reference, whose tile sizes are compile-time constants,
  * prepare_gso_task.cu:2503-2514).  sdgso_tune_variants() is the number of
- * compiled variants (filter form x gathers in flight per lane x
- * register/occupancy target).  sdgso_run_tune scores the current poses of
+ * compiled variants (ligand-block size x filter form x gathers in flight per
+ * lane x register/occupancy target); a name starting with "bcast2/" is a
+ * variant whose wave takes a 128-atom ligand block (two 64-atom blocks of one
+ * glowworm), the others take 64 atoms.  sdgso_run_tune scores the current poses of
  * each complex of the run with each variant `repeats` times (luciferin and
  * step counters untouched) and keeps, per complex, the fastest for later
  * steps: times_ms[n_complex][sdgso_tune_variants()] receives the
@@ -159,7 +161,8 @@ int sdgso_run_variants(sdgso_run r, int *variants);
 /* Filter statistics of the FP32 guard-band scoring kernel (no counterpart in
  * the reference, which scores every pair): scores the run's current poses
  * once with the chosen variant, leaves the run state unchanged, and
- * returns stats[6] = {waves (glowworm x ligand block), waves with a candidate
+ * returns stats[6] = {waves (glowworm x ligand block, or x pair of blocks
+ * for a two-block variant), waves with a candidate
  * receptor tile, candidate tiles visited, receptor atoms passing the atom
  * test, pairs queued for exact evaluation, pairs inside the cutoff}. */
 int sdgso_run_filter_stats(sdgso_run r, int64_t *stats);

@@ -146,6 +146,21 @@ static std::vector<int> group_order(const std::vector<double> &xyz, int n, int s
     return out;
 }
 
+// Ligand atoms in block order: 128-atom groups by recursive bisection, each
+// halved the same way, so that blocks 2j and 2j + 1 (64 atoms each) are the
+// two halves of one compact group, which the two-block kernels take together.
+static std::vector<int> block_order(const std::vector<double> &xyz, int n) {
+    const std::vector<int> g = group_order(xyz, n, 2 * kBlock);
+    std::vector<int> out;
+    out.reserve(n);
+    for (size_t lo = 0; lo < g.size(); lo += 2 * kBlock) {
+        const size_t hi = std::min(g.size(), lo + 2 * (size_t)kBlock);
+        std::vector<int> idx(g.begin() + lo, g.begin() + hi);
+        rcb(xyz, idx, 0, idx.size(), kBlock, out);
+    }
+    return out;
+}
+
 }  // namespace sd
 
 using namespace sd;
@@ -184,7 +199,7 @@ struct sdgso_run_s {
     DevBuf<DevComplex> dcx;
     DevBuf<int> swarm_cx;
     DevBuf<double> pos, luciferin, score, vision, ref, stream, partial;
-    DevBuf<int> nnei, moved, select, step, work;
+    DevBuf<int> nnei, moved, select, step, work, work2;
     DevBuf<int64_t> pairs;
     DevBuf<long long> partial_pairs;
     DevBuf<int> d_k;
@@ -194,9 +209,10 @@ struct sdgso_run_s {
     hipGraphExec_t graph = nullptr;
     bool profiling = false;
     // FP32 form: scoring-kernel variant of each complex (sdgso_run_tune) and
-    // the complex's range [work0[k], work0[k+1]) of the work list
+    // the complex's range [work0[k], work0[k+1]) of the work list (work2_0:
+    // of the list of block pairs that two-block variants read)
     std::vector<int> variants;
-    std::vector<size_t> work0;
+    std::vector<size_t> work0, work2_0;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_score, ev_move;
     double t_score = 0, t_move = 0;
     int64_t n_score = 0, n_move = 0;
@@ -314,7 +330,7 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
 static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<double> &msoa) {
     const int n = c->nl, nm = c->nml;
     std::vector<double> v(xyz, xyz + 3 * (size_t)n);
-    std::vector<int> order = group_order(v, n, kTile);
+    std::vector<int> order = block_order(v, n);
     const int nb = (n + kBlock - 1) / kBlock;
     std::vector<float4> blocks(nb);
     std::vector<int> perm((size_t)nb * kBlock, -1);
@@ -505,6 +521,15 @@ static ScoreArgs score_args(sdgso_run r) {
     return a;
 }
 
+// The work of complexes [k0, k1) in the list that variant v reads: (glowworm,
+// block) pairs, or (glowworm, first block of a pair) for two-block variants.
+static void set_work(sdgso_run r, int v, int k0, int k1, ScoreArgs &a) {
+    const bool two = score32_variant_blocks(v) == 2;
+    const std::vector<size_t> &w0 = two ? r->work2_0 : r->work0;
+    a.work = (two ? r->work2.p : r->work.p) + 2 * w0[k0];
+    a.n_work = (int)(w0[k1] - w0[k0]);
+}
+
 // FP32 scoring of the work list: consecutive complexes that use the same
 // variant share a launch.
 static void launch_score32_run(sdgso_run r, const ScoreArgs &a) {
@@ -513,8 +538,7 @@ static void launch_score32_run(sdgso_run r, const ScoreArgs &a) {
         int k1 = k0 + 1;
         while (k1 < n && r->variants[k1] == r->variants[k0]) k1++;
         ScoreArgs part = a;
-        part.work = a.work + 2 * r->work0[k0];
-        part.n_work = (int)(r->work0[k1] - r->work0[k0]);
+        set_work(r, r->variants[k0], k0, k1, part);
         HIPCHK(launch_score32(part, r->variants[k0], r->stream_h));
         k0 = k1;
     }
@@ -690,8 +714,9 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
         if (form == SDGSO_FP32_GUARD) {
             r->partial.alloc(N * (size_t)r->max_lblocks);
             r->partial_pairs.alloc(N * (size_t)r->max_lblocks);
-            // work list: (glowworm, ligand block) pairs, blocks of one glowworm adjacent
-            std::vector<int> work;
+            // work list: (glowworm, ligand block) pairs, blocks of one glowworm
+            // adjacent; work2: (glowworm, even block), the block pairs (b, b + 1)
+            std::vector<int> work, work2;
             size_t total = 0;
             for (int k = 0; k < n_complex; k++)
                 total += (size_t)n_swarms[k] * r->G * r->cxs[k]->dev.nlblocks;
@@ -699,7 +724,9 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
                 fail(SDGSO_EINVAL, "run too large: " + std::to_string(total) +
                                        " (glowworm, ligand block) pairs");
             work.reserve(2 * total);
+            work2.reserve(total + 2 * N);
             r->work0.assign(1, 0);
+            r->work2_0.assign(1, 0);
             size_t g0 = 0;
             for (int k = 0; k < n_complex; k++) {
                 const int nb = r->cxs[k]->dev.nlblocks;
@@ -708,12 +735,19 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
                     for (int b = 0; b < nb; b++) {
                         work.push_back((int)(g0 + g));
                         work.push_back(b);
+                        if (b % 2 == 0) {
+                            work2.push_back((int)(g0 + g));
+                            work2.push_back(b);
+                        }
                     }
                 g0 += ng;
                 r->work0.push_back(work.size() / 2);
+                r->work2_0.push_back(work2.size() / 2);
             }
             r->work.upload(work);
             r->work.n = work.size() / 2;
+            r->work2.upload(work2);
+            r->work2.n = work2.size() / 2;
             r->variants.assign(n_complex, score32_default_variant());
         }
         HIPCHK(hipStreamCreateWithFlags(&r->own_stream, hipStreamNonBlocking));
@@ -913,11 +947,10 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
         HIPCHK(hipEventCreate(&e1));
         for (int k = 0; k < nc; k++) {
             ScoreArgs part = a;
-            part.work = a.work + 2 * r->work0[k];
-            part.n_work = (int)(r->work0[k + 1] - r->work0[k]);
             double best = 1e300;
             int best_v = 0;
             for (int v = 0; v < nv; v++) {
+                set_work(r, v, k, k + 1, part);
                 HIPCHK(launch_score32(part, v, r->stream_h));  // warm-up
                 HIPCHK(hipEventRecord(e0, r->stream_h));
                 for (int i = 0; i < repeats; i++) HIPCHK(launch_score32(part, v, r->stream_h));

@@ -134,6 +134,7 @@ struct WaveCtx {
     double cb[3];     // block centre (fixed-point origin)
     float fc[3];
     float reach;      // cutoff + block radius + displacement bounds
+    float dlr;        // bound of the mode displacements dl + dr (A)
     float band;       // this wave's guard band (A^2): the run's, widened to the error bound
     float rmax;       // bound of |r| for the receptor atoms that pass the atom test
     int4 L;           // this lane's ligand atom: fixed point + type*20
@@ -141,9 +142,13 @@ struct WaveCtx {
 };
 
 // Candidate tiles from tile t0 on: 64 sphere tests per instruction, listed
-// in LDS (at most `cap` entries); advances t0 past the tiles tested.
+// in LDS (at most `cap` entries); advances t0 past the tiles tested.  With
+// kTwo a tile is also a candidate when it reaches the second block's sphere
+// (centre fb, reach rb; rb < 0: no second block).
+template <bool kTwo = false>
 __device__ __forceinline__ int scan_candidates(int ntiles, const float4 *tiles, const WaveCtx &x,
-                                               unsigned short *ct, int &t0, int cap) {
+                                               unsigned short *ct, int &t0, int cap,
+                                               const float *fb = nullptr, float rb = -1.f) {
     int n = 0;
     for (; t0 < ntiles && n <= cap - 64; t0 += 64) {
         const int t = t0 + x.lane;
@@ -153,6 +158,11 @@ __device__ __forceinline__ int scan_candidates(int ntiles, const float4 *tiles, 
             const float tx = tl.x - x.fc[0], ty = tl.y - x.fc[1], tz = tl.z - x.fc[2];
             const float lim = x.reach + tl.w;
             ok = tx * tx + ty * ty + tz * tz <= lim * lim;
+            if constexpr (kTwo) {
+                const float ux = tl.x - fb[0], uy = tl.y - fb[1], uz = tl.z - fb[2];
+                const float limb = rb + tl.w;
+                ok = ok || (rb >= 0.f && ux * ux + uy * uy + uz * uz <= limb * limb);
+            }
         }
         const unsigned long long m = __ballot(ok);
         if (ok)
@@ -171,10 +181,11 @@ __device__ __forceinline__ int scan_candidates(const DevComplex &c, const WaveCt
 }
 
 // Set-up common to the variants: pose, block sphere, ligand atom, receptor
-// extents (LDS), candidate tiles (LDS).  Returns false for an idle wave.
+// extents (LDS), candidate tiles (LDS; left to the caller when scan is
+// false).  Returns false for an idle wave.
 __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex &c, int gid,
                                            int b, float *ef, unsigned short *ct, WaveCtx &x,
-                                           int cap = kMaxTiles) {
+                                           int cap = kMaxTiles, bool scan = true) {
     x.lane = threadIdx.x & 63;
     x.gid = gid;
     x.b = b;
@@ -216,13 +227,14 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     // in mode order).  With |dx| + |dy| + |dz| <= 15 sqrt(3) and 3 u 225 for
     // the FP32 sum of squares, |d2 - d2_ref| <= 9.04e-5 + 3.098e-6 k S A^2.
     x.band = fmaxf(a.band, 9.1e-5f + 3.11e-6f * (float)max(11, c.nmr + 2) * dr);
+    x.dlr = dl + dr;
     x.reach = 15.0f + blk.w + dl + dr + 0.01f;
     x.rmax = x.reach + dr + 0.01f;
     x.fc[0] = (float)x.cb[0];
     x.fc[1] = (float)x.cb[1];
     x.fc[2] = (float)x.cb[2];
     x.t_scan = 0;
-    x.ncand = scan_candidates(c, x, ct, x.t_scan, cap);
+    x.ncand = scan ? scan_candidates(c, x, ct, x.t_scan, cap) : 0;
     return true;
 }
 
@@ -272,10 +284,13 @@ __device__ __forceinline__ bool tile_atom(const DevComplex &c, const WaveCtx &x,
     return nearb;
 }
 
+// The wave's energy and pair count go to block x.b's partial; a second block
+// b2 (>= 0) of a two-block wave gets zeros.
 __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
                                                const Kahan &acc, int npairs,
                                                unsigned int nguard, long long n_cand,
-                                               long long n_near, long long n_queued) {
+                                               long long n_near, long long n_queued,
+                                               int b2 = -1) {
     double e = (double)acc.sum - (double)acc.comp;
     long long np = npairs;
     for (int off = 32; off > 0; off >>= 1) {
@@ -287,6 +302,10 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
     if (x.lane == 0) {
         a.partial[(size_t)x.gid * a.max_lblocks + x.b] = e;
         a.partial_pairs[(size_t)x.gid * a.max_lblocks + x.b] = np;
+        if (b2 >= 0) {
+            a.partial[(size_t)x.gid * a.max_lblocks + b2] = 0.0;
+            a.partial_pairs[(size_t)x.gid * a.max_lblocks + b2] = 0;
+        }
         if (a.guard_count && ng) atomicAdd(a.guard_count, (unsigned long long)ng);
         if (a.stats) {
             // counters spread over kStatSlots slots each, so that waves rarely collide
@@ -629,6 +648,15 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 // Rows past the batch and padding ligand lanes carry a norm of 60000, so they
 // never pass; a wave whose atoms could exceed the half range (|r| >= 250 A)
 // takes the FP32 pre-test.
+// Two ligand blocks per wave (kLB = 2): the wave takes blocks b and b + 1 of
+// one glowworm (a 128-atom ligand block, two atoms per lane), so the
+// candidate scan, the tile visits and the displacement of the receptor atoms
+// near both blocks are shared.  Both blocks' atoms are rounded to the grid of
+// block b's centre.  A pending atom carries a near flag per block; a batch is
+// packed in the order [near A only | near both | near B only], and the
+// pre-test makes one pass per block over its range, so every atom is tested
+// only against the blocks it is near.  rho, the bound of |r| and of |l| + 15
+// about the origin, takes the second block's sphere into account.
 // Queue room: kUnroll rounds of 64 pairs left over from a flush, plus the
 // pairs of four pre-tested atoms (or of one MFMA 32 x 32 group).
 template <int kUnroll, bool kMfma>
@@ -687,16 +715,19 @@ __device__ __forceinline__ T *uniform_ptr(T *p) {
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
-template <int kUnroll, int kMinWaves, bool kMfma, int kWpb, int kTilesB, int kC>
+template <int kUnroll, int kMinWaves, bool kMfma, int kWpb, int kTilesB, int kC, int kLB = 1>
 __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArgs a) {
+    static_assert(kLB == 1 || (kLB == 2 && !kMfma), "one block, or two with the FP32 pre-test");
     constexpr int kPend = 64 * kTilesB + 64;  // pending near slots: a batch's worth + one round
     constexpr int kQueue = queue_len<kUnroll, kMfma>();
     constexpr int kGroup = 64 * kUnroll;  // pairs of one group of rounds
+    // a pending entry: receptor slot (low 16 bits) and, with two blocks, near flags
+    using PdT = std::conditional_t<kLB == 2, unsigned, unsigned short>;
     __shared__ int4 s_rt[kWpb][kRtRows];     // packed near receptor atoms: fixed point + row
     __shared__ float4 s_rf[kWpb][64];        //   this batch's atoms in A: x, y, z, |r|^2
     __shared__ unsigned short s_rs[kWpb][2 * 64];  // receptor slots of the packed atoms
-    __shared__ unsigned short s_pd[kWpb][kPend];   // near slots waiting for displacement
-    __shared__ int4 s_lg[kWpb][64];          // the block's ligand atoms: fixed point + type*20
+    __shared__ PdT s_pd[kWpb][kPend];        // near slots waiting for displacement
+    __shared__ int4 s_lg[kWpb][64 * kLB];    // the blocks' ligand atoms: fixed point + type*20
     __shared__ int s_q[kWpb][kQueue];        // queued pairs (LDS offsets of both atoms)
     __shared__ float s_ef[kWpb][SDGSO_MAX_MODES];
     __shared__ unsigned short s_ct[kWpb][kCandB];  // candidate tiles, scanned in chunks
@@ -719,36 +750,80 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     int4 *rt = s_rt[w];
     float4 *rf = s_rf[w];
     unsigned short *rs = s_rs[w];
-    unsigned short *pd = s_pd[w];
+    PdT *pd = s_pd[w];
     int4 *lg = s_lg[w];
     int *qq = s_q[w];
     float *ef = s_ef[w];
     unsigned short *ct = s_ct[w];
     WaveCtx x;
-    wave_setup(a, c, gid, b, ef, ct, x, kCandB);
+    wave_setup(a, c, gid, b, ef, ct, x, kCandB, kLB == 1);
     const int lane = x.lane;
     lg[lane] = x.L;
+    // second block (kLB = 2): b + 1 when the glowworm has it; its sphere (fb,
+    // reach_b < 0 without it) and this lane's atom on block b's grid
+    int b2 = -1;
+    float fb[3] = {0.f, 0.f, 0.f};
+    float reach_b = -1.f;
+    int4 LB = make_int4(-kFar, -kFar, -kFar, 0);
+    int lig_b = -1;
+    float rho = x.reach;  // bound of |r| and of |l| + 15 about the origin
+    if constexpr (kLB == 2) {
+        b2 = b + 1 < __builtin_amdgcn_readfirstlane(c.nlblocks) ? b + 1 : -1;
+        if (b2 >= 0) {
+            double inv[4];
+            quat_inverse(x.pose + 3, inv);
+            const float4 blk = gld(c.lblocks, b2);
+            double cx = blk.x, cy = blk.y, cz = blk.z;
+            rotate_point(x.pose + 3, inv, cx, cy, cz);
+            cx += x.pose[0];
+            cy += x.pose[1];
+            cz += x.pose[2];
+            lig_b = gld(c.lperm, b2 * kBlock + lane);
+            if (lig_b >= 0) {
+                double px, py, pz;
+                ligand_slot64(c, x.pose, inv, b2, lane, px, py, pz);
+                LB = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
+                               gld(c.ltype, lig_b));
+            }
+            fb[0] = uniform_f32((float)cx);
+            fb[1] = uniform_f32((float)cy);
+            fb[2] = uniform_f32((float)cz);
+            reach_b = uniform_f32(15.0f + blk.w + x.dlr + 0.01f);
+            const float dx = fb[0] - x.fc[0], dy = fb[1] - x.fc[1], dz = fb[2] - x.fc[2];
+            rho = fmaxf(rho, __builtin_sqrtf(dx * dx + dy * dy + dz * dz) * 1.0001f + reach_b + 0.01f);
+        }
+        lg[64 + lane] = LB;
+        x.ncand = scan_candidates<true>(ntiles, tiles, x, ct, x.t_scan, kCandB, fb, reach_b);
+    }
     // the set-up's wave-uniform results, in scalar registers
     const double cb0 = uniform_f64(x.cb[0]), cb1 = uniform_f64(x.cb[1]), cb2 = uniform_f64(x.cb[2]);
     const float fc0 = uniform_f32(x.fc[0]), fc1 = uniform_f32(x.fc[1]), fc2 = uniform_f32(x.fc[2]);
     const float reach = uniform_f32(x.reach);
+    rho = uniform_f32(rho);
     if (lane == 0) rt[2 * 64] = make_int4(kFar, kFar, kFar, 0);
     wave_sync_lds();
     // queue words: LDS byte offsets from s_rt[0][0] (high half) and s_lg[0][0] (low half)
     const char *rt_base = (const char *)&s_rt[0][0];
     const char *lg_base = (const char *)&s_lg[0][0];
     const unsigned rt0 = (unsigned)(w * kRtRows) * 16u;
-    const unsigned lane_lo = (unsigned)(w * 64 + lane) * 16u;
-    const int null_tag = (int)(((rt0 + 2 * 64 * 16u) << 16) | (unsigned)(w * 64) * 16u);
+    const unsigned lane_lo = (unsigned)(w * 64 * kLB + lane) * 16u;
+    const int null_tag = (int)(((rt0 + 2 * 64 * 16u) << 16) | (unsigned)(w * 64 * kLB) * 16u);
     const float band = uniform_f32(x.band);
     const float bandq = band * kQ2;
-    const float thr = 225.0f + 2.0f * band + 0x1p-18f * reach * reach + 1e-3f;
+    const float thr = 225.0f + 2.0f * band + 0x1p-18f * rho * rho + 1e-3f;
     const float reach2 = reach * reach;
+    const float reach2b = reach_b >= 0.f ? reach_b * reach_b : -1.f;
     // this lane's pre-test operands: -2 l and T - |l|^2 (padding lanes never pass)
     const float lx = (float)x.L.x * kInvQ, ly = (float)x.L.y * kInvQ, lz = (float)x.L.z * kInvQ;
     const float m2x = -2.f * lx, m2y = -2.f * ly, m2z = -2.f * lz;
     const float tl = x.lig >= 0 ? thr - __builtin_fmaf(lz, lz, __builtin_fmaf(ly, ly, lx * lx))
                                 : -1e30f;
+    // and for the second block's atom
+    const float lbx = (float)LB.x * kInvQ, lby = (float)LB.y * kInvQ, lbz = (float)LB.z * kInvQ;
+    const float m2xb = -2.f * lbx, m2yb = -2.f * lby, m2zb = -2.f * lbz;
+    const float tlb = lig_b >= 0
+                          ? thr - __builtin_fmaf(lbz, lbz, __builtin_fmaf(lby, lby, lbx * lbx))
+                          : -1e30f;
     // MFMA pre-test: B operands of the two 32-atom column groups, and the threshold
     const bool mfma_ok = kMfma && uniform_f32(x.rmax) < 250.0f;
     const float thr16 = 225.0f + 2.0f * band + 0x1p-8f * reach * (reach - 15.0f) +
@@ -798,7 +873,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         ring[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tbl, off, 0, 0));
         npairs += pc.take ? 1 : 0;
         if (pc.near) {
-            const int li = (int)((unsigned)(tag & 0xffff) >> 4) - w * 64;
+            // ligand slot li of block x.b (li >= 64: slot li - 64 of block x.b + 1)
+            const int li = (int)((unsigned)(tag & 0xffff) >> 4) - w * 64 * kLB;
             const int ki = (int)((unsigned)tag >> 20) - w * kRtRows;
             const float2 g2 = guard_pair64(c, x.pose, gld(c.lperm, x.b * kBlock + li),
                                            gld(c.rperm, (int)rs[ki]));
@@ -858,13 +934,17 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // queued at the end are evaluated too, since the next batch overwrites it.
     auto batch = [&](int n) {
         old = qn;
-        // this lane's atom: slot, tile centre, coordinates and type row (loads first)
+        // this lane's atom: slot, near flags, tile centre, coordinates and type
+        // row (loads first)
         int slot = 0, row = 0;
+        unsigned fl = 0;
         float4 tc = make_float4(0.f, 0.f, 0.f, 0.f);
         double2 rxy = make_double2(0.0, 0.0);
         double rz = 0.0;
         if (lane < n) {
-            slot = pd[lane];
+            const unsigned e = pd[lane];
+            slot = (int)(e & 0xffffu);
+            fl = e >> 16;
             tc = gld(tiles, slot / kTile);
             rxy = gld2(rrel4, 2 * (size_t)slot);
             rz = gld(rrel4, 4 * (size_t)slot + 2);
@@ -897,7 +977,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 for (int r = 0; r < 4; r++) {
                     av[r] = (k0 + r) * kApr + lane / kC;
                     q[r] = (k0 + r < nround && av[r] < n)
-                               ? gld(rmodA, (size_t)pd[av[r]] * kC + j)
+                               ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * kC + j)
                                : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
 #pragma unroll
@@ -922,15 +1002,34 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             wave_sync_lds();
             if (lane < n) D = rf[lane];
         }
+        // packing position: the lane's own, or with two blocks, the batch
+        // ordered as [near block A only | near both | near block B only], so
+        // that each block's atoms form one range, [0, a_end) and [b_begin, n)
+        int pos = lane, a_end = n, b_begin = 0;
+        if constexpr (kLB == 2) {
+            const unsigned long long m_a = __ballot(lane < n && fl == 1u);
+            const unsigned long long m_ab = __ballot(lane < n && fl == 3u);
+            const unsigned long long m_b = __ballot(lane < n && fl == 2u);
+            const int n_a = __popcll(m_a), n_ab = __popcll(m_ab);
+            const unsigned long long mine = fl == 1u ? m_a : (fl == 3u ? m_ab : m_b);
+            pos = (fl == 1u ? 0 : (fl == 3u ? n_a : n_a + n_ab)) +
+                  (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32),
+                                                 __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+            a_end = n_a + n_ab;
+            b_begin = n_a;
+            wave_sync_lds();  // every lane has read its staged displacement
+        }
         if (lane < n) {
             const int4 R = make_int4(qround(((double)tc.x - cb0) + rxy.x + (double)D.x),
                                      qround(((double)tc.y - cb1) + rxy.y + (double)D.y),
                                      qround(((double)tc.z - cb2) + rz + (double)D.z), row);
             const float fx = (float)R.x * kInvQ, fy = (float)R.y * kInvQ, fz = (float)R.z * kInvQ;
-            rt[64 * p + lane] = R;
-            rf[lane] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
-            rs[64 * p + lane] = (unsigned short)slot;
+            rt[64 * p + pos] = R;
+            rf[pos] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
+            rs[64 * p + pos] = (unsigned short)slot;
         }
+        // a round past the batch's end reads a far-away atom
+        if (kLB == 2 && lane >= n) rf[lane] = make_float4(0.f, 0.f, 0.f, 1e30f);
         wave_sync_lds();
         const unsigned hib = rt0 + 64u * 16u * (unsigned)p;  // offset of this buffer's atom 0
         if (mfma_ok) {
@@ -977,6 +1076,42 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                     }
                 }
             }
+        } else if constexpr (kLB == 2) {
+            // a pass per block over its range, four packed atoms per round;
+            // the ranges are widened to whole rounds (an atom outside a
+            // block's reach has no pair that counts with it, and rf[n..63]
+            // holds a far-away atom that never passes)
+            auto pretest = [&](int i_begin, int i_end, float ox, float oy, float oz, float ot,
+                               unsigned olo) {
+                for (int i0 = i_begin; i0 < i_end; i0 += 4) {
+                    float4 r[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) r[j] = rf[i0 + j];
+                    unsigned long long m[4];
+                    bool in[4];
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        const float wv2 = __builtin_fmaf(
+                            r[j].x, ox, __builtin_fmaf(r[j].y, oy, __builtin_fmaf(r[j].z, oz, r[j].w)));
+                        in[j] = wv2 <= ot;
+                        m[j] = __ballot(in[j]);
+                    }
+#pragma unroll
+                    for (int j = 0; j < 4; j++) {
+                        if (m[j]) {
+                            if (in[j])
+                                qq[qn + __builtin_amdgcn_mbcnt_hi(
+                                            (unsigned)(m[j] >> 32),
+                                            __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
+                                    (int)(((hib + 16u * (unsigned)(i0 + j)) << 16) | olo);
+                            qn += __popcll(m[j]);
+                        }
+                    }
+                    if (qn > kQueue - 4 * 64) flush_groups();
+                }
+            };
+            pretest(0, a_end, m2x, m2y, m2z, tl, lane_lo);
+            pretest(b_begin & ~3, n, m2xb, m2yb, m2zb, tlb, lane_lo + 64u * 16u);
         } else {
             // four packed atoms per round: their broadcast reads are in flight together
             for (int i0 = 0; i0 < n; i0 += 4) {
@@ -1015,18 +1150,28 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // against the block sphere; near slots join the pending list.
     int npd = 0;
     long long n_cand = 0;
-    // near test of this lane's atom of tile t (position pp); near slots are appended
+    // near test of this lane's atom of tile t (position pp); near slots are
+    // appended (with two blocks, with a near flag per block)
     auto visit = [&](int t, float4 pp) {
         const float ax = pp.x - fc0;
         const float ay = pp.y - fc1;
         const float az = pp.z - fc2;
-        const bool nearb = ax * ax + ay * ay + az * az <= reach2;
+        bool nearb = ax * ax + ay * ay + az * az <= reach2;
+        unsigned fl = 0;
+        if constexpr (kLB == 2) {
+            const float bx = pp.x - fb[0];
+            const float by = pp.y - fb[1];
+            const float bz = pp.z - fb[2];
+            const bool nb = bx * bx + by * by + bz * bz <= reach2b;
+            fl = (nearb ? 1u : 0u) | (nb ? 2u : 0u);
+            nearb = nearb || nb;
+        }
         const unsigned long long m = __ballot(nearb);
         if (!m) return;
         if (nearb)
             pd[npd + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
-                (unsigned short)(t * kTile + lane);
+                (PdT)((unsigned)(t * kTile + lane) | (fl << 16));
         npd += __popcll(m);
         n_near += __popcll(m);
     };
@@ -1057,10 +1202,10 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 batch(64);
                 // move the remaining pending slots to the front
                 const int rem = npd - 64;
-                unsigned short v[kTilesB];
+                PdT v[kTilesB];
 #pragma unroll
                 for (int j = 0; j < kTilesB; j++)
-                    v[j] = 64 * j + lane < rem ? pd[64 + 64 * j + lane] : (unsigned short)0;
+                    v[j] = 64 * j + lane < rem ? pd[64 + 64 * j + lane] : (PdT)0;
                 wave_sync_lds();
 #pragma unroll
                 for (int j = 0; j < kTilesB; j++)
@@ -1070,7 +1215,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         }
         if (x.t_scan >= ntiles) break;
         __builtin_amdgcn_wave_barrier();  // every lane has read the list before it is rewritten
-        x.ncand = scan_candidates(ntiles, tiles, x, ct, x.t_scan, kCandB);
+        x.ncand = scan_candidates<kLB == 2>(ntiles, tiles, x, ct, x.t_scan, kCandB, fb, reach_b);
     }
     if (npd) {
         wave_sync_lds();
@@ -1079,7 +1224,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     flush_all();
 #pragma unroll
     for (int j = 0; j < kUnroll; j++) acc.add(ring[j], true);
-    write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued);
+    write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued, b2);
 }
 
 // Per glowworm: energy = partials in block order (FP64), score, luciferin,
@@ -1123,9 +1268,13 @@ struct Variant {
     const char *name;
     void (*kernel[3])(ScoreArgs);  // for 8, 16 and 32 float4 of modes per receptor slot
     int waves;                     // waves per workgroup
+    int blocks = 1;                // ligand blocks per wave (work list of block pairs for 2)
 };
 #define SD_BCAST(R, O, M, T) \
     { score32b_kernel<R, O, M, 1, T, 8>, score32b_kernel<R, O, M, 1, T, 16>, score32b_kernel<R, O, M, 1, T, 32> }
+#define SD_BCAST2(R, O, T) \
+    { score32b_kernel<R, O, false, 1, T, 8, 2>, score32b_kernel<R, O, false, 1, T, 16, 2>, \
+      score32b_kernel<R, O, false, 1, T, 32, 2> }
 #define SD_SAME(K) \
     { K, K, K }
 const Variant kVariants[] = {
@@ -1134,6 +1283,7 @@ const Variant kVariants[] = {
     {"bcast/ring4/occ4", SD_BCAST(4, 4, false, 4), 1},
     {"bcast/ring2/occ5", SD_BCAST(2, 5, false, 4), 1},
     {"bcast/ring4/occ5", SD_BCAST(4, 5, false, 4), 1},
+    {"bcast2/ring4/occ4", SD_BCAST2(4, 4, 4), 1, 2},
     {"readlane/unroll4/occ4", SD_SAME((score32r_kernel<4, 4>)), kWaves},
     {"readlane/unroll4/occ3", SD_SAME((score32r_kernel<4, 3>)), kWaves},
     {"mfma/unroll4/occ3", SD_SAME((score32m_kernel<4, 3>)), kWaves},
@@ -1150,6 +1300,8 @@ int score32_default_variant() {
         if (std::string(kVariants[v].name) == "bcast/ring4/occ4") return v;
     return 0;
 }
+
+int score32_variant_blocks(int v) { return (v >= 0 && v < kNumVariants) ? kVariants[v].blocks : 1; }
 
 const char *score32_variant_name(int v) {
     return (v >= 0 && v < kNumVariants) ? kVariants[v].name : "?";

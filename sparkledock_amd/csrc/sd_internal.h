@@ -150,6 +150,7 @@ hipError_t launch_score64(const ScoreArgs &a, int max_nl, hipStream_t s);
 hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s);
 int score32_variants();
 const char *score32_variant_name(int v);
+int score32_variant_blocks(int v);  // ligand blocks per wave: 1, or 2 (work list of block pairs)
 int score32_default_variant();
 hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s);
 hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s);

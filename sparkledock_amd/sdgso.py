@@ -330,7 +330,8 @@ class Run:
 
     def filter_stats(self):
         """Counts of one FP32 scoring pass over the current poses: waves
-        (glowworm x ligand block), waves with candidate tiles, candidate tiles,
+        (glowworm x ligand block, or x block pair for a two-block variant),
+        waves with candidate tiles, candidate tiles,
         receptor atoms passing the atom test, queued pairs, in-cutoff pairs."""
         s = np.zeros(6, np.int64)
         _check(lib().sdgso_run_filter_stats(self.h, _p(s, _I64)))
