@@ -187,8 +187,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ranking-steps", type=int, default=30,
                     help="steps of the end-to-end FP32-vs-FP64 ranking comparison on c1 (0 = skip)")
-    ap.add_argument("--variant", type=int, default=None,
-                    help="FP32 scoring-kernel variant to use instead of the autotuner's choice")
+    ap.add_argument("--variant", default=None,
+                    help="FP32 scoring-kernel variant (name, e.g. bcast2/ring4/occ4, or index) "
+                         "to use instead of the autotuner's choice")
     ap.add_argument("--profile-step", action="store_true",
                     help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
                          "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE, "
@@ -204,6 +205,15 @@ def main():
         sys.exit("no GPU visible")
     torch.cuda.set_device(0)
     stream = torch.cuda.current_stream()
+    # the library is loaded after torch has set up the device
+    if args.variant is not None:
+        names = sdgso.tune_variants()
+        if args.variant.isdigit() and int(args.variant) < len(names):
+            args.variant = int(args.variant)
+        elif args.variant in names:
+            args.variant = names.index(args.variant)
+        else:
+            sys.exit("unknown variant %r; compiled variants: %s" % (args.variant, ", ".join(names)))
 
     if args.profile_step:
         x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
@@ -307,7 +317,7 @@ def main():
     for v, w in zip(chosen_names, blocks):
         share[v] = share.get(v, 0) + w
     variant_name = max(share, key=share.get)
-    kernel_name = "score32%s_kernel" % {"bcast": "b", "readlane": "r", "mfma": "m"}.get(
+    kernel_name = "score32%s_kernel" % {"bcast": "b", "bcast2": "b", "readlane": "r", "mfma": "m"}.get(
         variant_name.split("/")[0].split("-")[0], "")
     out = {
         "metric": METRIC,
