@@ -662,6 +662,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 template <int kUnroll, bool kMfma>
 constexpr int queue_len() { return 64 * kUnroll + (kMfma ? 1024 : 256); }
 constexpr int kCandB = 128;     // candidate-list chunk of the broadcast variant
+constexpr int kDispRounds = 8;  // displacement rounds whose mode-line loads are in flight together
 constexpr int kRtRows = 2 * 64 + 1;        // two batches of packed atoms and one far-away atom
 constexpr unsigned kNoLoad = 0x80000000u;  // buffer offset past the table: the load returns 0
 
@@ -955,7 +956,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         // atom's modes are kC float4 in whole 128-byte lines, so a round reads
         // 64 / kC lines), weighs its four floats by their modes' extents per
         // component, and the group's sums reach its last lane; they are staged
-        // in rf, which the packing below overwrites
+        // in rf, which the packing below overwrites.  The loads of kDispRounds
+        // rounds are issued together.
         float4 D = make_float4(0.f, 0.f, 0.f, 0.f);
         if (nmr > 0) {
             const int j = lane & (kC - 1);
@@ -970,18 +972,18 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             }
             constexpr int kApr = 64 / kC;  // atoms per round
             const int nround = (n + kApr - 1) / kApr;
-            for (int k0 = 0; k0 < nround; k0 += 4) {
-                float4 q[4];
-                int av[4];
+            for (int k0 = 0; k0 < nround; k0 += kDispRounds) {
+                float4 q[kDispRounds];
+                int av[kDispRounds];
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
+                for (int r = 0; r < kDispRounds; r++) {
                     av[r] = (k0 + r) * kApr + lane / kC;
                     q[r] = (k0 + r < nround && av[r] < n)
                                ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * kC + j)
                                : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
 #pragma unroll
-                for (int r = 0; r < 4; r++) {
+                for (int r = 0; r < kDispRounds; r++) {
                     if (k0 + r >= nround) break;
                     float px = q[r].x * wx[0], py = q[r].x * wy[0], pz = q[r].x * wz[0];
                     px = __builtin_fmaf(q[r].y, wx[1], px);
