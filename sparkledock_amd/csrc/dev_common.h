@@ -54,6 +54,39 @@ __device__ __forceinline__ void rotate_point(const double q[4], const double inv
     z = b0 * inv[3] + b1 * inv[2] - b2 * inv[1] + b3 * inv[0];
 }
 
+// Rotation matrix of p -> q (0, p) q^-1 for a quaternion of any norm.  The
+// FP32 kernels rotate with it only the positions that they round to their
+// 2^-20 A grid, where its last-bit differences from rotate_point's order are
+// far below a grid unit; every pair decided in FP64 goes through
+// rotate_point.
+struct Rot64 {
+    double m[9];
+};
+
+__device__ __forceinline__ Rot64 rot_of(const double q[4]) {
+    const double w = q[0], x = q[1], y = q[2], z = q[3];
+    const double ww = w * w, xx = x * x, yy = y * y, zz = z * z;
+    const double s = 1.0 / (ww + xx + yy + zz), s2 = 2.0 * s;
+    Rot64 r;
+    r.m[0] = (ww + xx - yy - zz) * s;
+    r.m[1] = (x * y - w * z) * s2;
+    r.m[2] = (x * z + w * y) * s2;
+    r.m[3] = (x * y + w * z) * s2;
+    r.m[4] = (ww - xx + yy - zz) * s;
+    r.m[5] = (y * z - w * x) * s2;
+    r.m[6] = (x * z - w * y) * s2;
+    r.m[7] = (y * z + w * x) * s2;
+    r.m[8] = (ww - xx - yy + zz) * s;
+    return r;
+}
+
+__device__ __forceinline__ void rot_apply(const Rot64 &r, double &x, double &y, double &z) {
+    const double a = x, b = y, c = z;
+    x = fma(r.m[0], a, fma(r.m[1], b, r.m[2] * c));
+    y = fma(r.m[3], a, fma(r.m[4], b, r.m[5] * c));
+    z = fma(r.m[6], a, fma(r.m[7], b, r.m[8] * c));
+}
+
 // dist_to_bins[d] - 1 (interface.h:30-35) for d in [0, 29]:
 // 0,0,0,1,...,13 (d = 3..15), then pairs 13,14,14,...,19,19, then 20 at d = 29.
 __device__ __forceinline__ int bin_of_d(int d) { return max(0, min(d - 2, (d + 11) >> 1)); }
@@ -117,10 +150,13 @@ __device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double 
 }
 
 // The same for slot `lane` of ligand block b, read from the block-slot copy
-// (every load of a wave is contiguous); the arithmetic is ligand_atom64's.
-__device__ __forceinline__ void ligand_slot64(const DevComplex &c, const double *pose,
-                                              const double inv[4], int b, int lane, double &x,
-                                              double &y, double &z) {
+// (every load of a wave is contiguous), for a position that the FP32 kernels
+// round to their grid: the modes are added with FMAs and the rotation is the
+// matrix R of the pose, so the result is within a few FP64 ulps of
+// ligand_atom64's.
+__device__ __forceinline__ void ligand_slot_grid(const DevComplex &c, const double *pose,
+                                                 const Rot64 &R, int b, int lane, double &x,
+                                                 double &y, double &z) {
     const int nm = c.nml;
     const double *B = c.lblk + (size_t)b * (3 + 3 * nm) * kBlock + lane;
     x = gld(B, 0);
@@ -140,19 +176,19 @@ __device__ __forceinline__ void ligand_slot64(const DevComplex &c, const double 
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            x += mx[k] * e[k];
-            y += my[k] * e[k];
-            z += mz[k] * e[k];
+            x = fma(mx[k], e[k], x);
+            y = fma(my[k], e[k], y);
+            z = fma(mz[k], e[k], z);
         }
     }
     for (; m < nm; m++) {
         const double *M = B + (size_t)(3 + 3 * m) * kBlock;
         const double e = ext[m];
-        x += gld(M, 0) * e;
-        y += gld(M, kBlock) * e;
-        z += gld(M, 2 * kBlock) * e;
+        x = fma(gld(M, 0), e, x);
+        y = fma(gld(M, kBlock), e, y);
+        z = fma(gld(M, 2 * kBlock), e, z);
     }
-    rotate_point(pose + 3, inv, x, y, z);
+    rot_apply(R, x, y, z);
     x += pose[0];
     y += pose[1];
     z += pose[2];

@@ -136,6 +136,7 @@ struct WaveCtx {
     float reach;      // cutoff + block radius + displacement bounds
     float dlr;        // bound of the mode displacements dl + dr (A)
     float band;       // this wave's guard band (A^2): the run's, widened to the error bound
+    Rot64 rot;        // the pose's rotation matrix (set-up only)
     float rmax;       // bound of |r| for the receptor atoms that pass the atom test
     int4 L;           // this lane's ligand atom: fixed point + type*20
     int lig;          // its original index (-1 = padding)
@@ -192,13 +193,12 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     x.pose = a.pos + (size_t)gid * a.P;
     const double *rext = x.pose + 7;
     const double *lext = x.pose + 7 + c.nmr;
-    double inv[4];
-    quat_inverse(x.pose + 3, inv);
+    x.rot = rot_of(x.pose + 3);
     const float4 blk = gld(c.lblocks, b);
     x.cb[0] = blk.x;
     x.cb[1] = blk.y;
     x.cb[2] = blk.z;
-    rotate_point(x.pose + 3, inv, x.cb[0], x.cb[1], x.cb[2]);
+    rot_apply(x.rot, x.cb[0], x.cb[1], x.cb[2]);
     x.cb[0] += x.pose[0];
     x.cb[1] += x.pose[1];
     x.cb[2] += x.pose[2];
@@ -207,7 +207,7 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     x.L = make_int4(-kFar, -kFar, -kFar, 0);
     if (x.lig >= 0) {
         double px, py, pz;
-        ligand_slot64(c, x.pose, inv, b, x.lane, px, py, pz);
+        ligand_slot_grid(c, x.pose, x.rot, b, x.lane, px, py, pz);
         x.L = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
                         gld(c.ltype, x.lig));
     }
@@ -771,18 +771,16 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     if constexpr (kLB == 2) {
         b2 = b + 1 < __builtin_amdgcn_readfirstlane(c.nlblocks) ? b + 1 : -1;
         if (b2 >= 0) {
-            double inv[4];
-            quat_inverse(x.pose + 3, inv);
             const float4 blk = gld(c.lblocks, b2);
             double cx = blk.x, cy = blk.y, cz = blk.z;
-            rotate_point(x.pose + 3, inv, cx, cy, cz);
+            rot_apply(x.rot, cx, cy, cz);
             cx += x.pose[0];
             cy += x.pose[1];
             cz += x.pose[2];
             lig_b = gld(c.lperm, b2 * kBlock + lane);
             if (lig_b >= 0) {
                 double px, py, pz;
-                ligand_slot64(c, x.pose, inv, b2, lane, px, py, pz);
+                ligand_slot_grid(c, x.pose, x.rot, b2, lane, px, py, pz);
                 LB = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
                                gld(c.ltype, lig_b));
             }
