@@ -653,7 +653,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
 // candidate scan, the tile visits and the displacement of the receptor atoms
 // near both blocks are shared.  Both blocks' atoms are rounded to the grid of
 // block b's centre.  A pending atom carries a near flag per block; a batch is
-// packed in the order [near A only | near both | near B only], and the
+// packed in the order [near b only | near both | near b + 1 only], and the
 // pre-test makes one pass per block over its range, so every atom is tested
 // only against the blocks it is near.  rho, the bound of |r| and of |l| + 15
 // about the origin, takes the second block's sphere into account.
@@ -1003,7 +1003,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             if (lane < n) D = rf[lane];
         }
         // packing position: the lane's own, or with two blocks, the batch
-        // ordered as [near block A only | near both | near block B only], so
+        // ordered as [near block b only | near both | near block b + 1 only], so
         // that each block's atoms form one range, [0, a_end) and [b_begin, n)
         int pos = lane, a_end = n, b_begin = 0;
         if constexpr (kLB == 2) {
