@@ -91,6 +91,25 @@ def per_bin_counts(pr, poses, form, variant=None):
     return np.array(out)
 
 
+@pytest.mark.parametrize("nl", [40, 64, 128, 129, 200, 256])
+def test_ligand_block_counts(nl):
+    """Ligands of one to four 64-atom blocks, full and partial: the two-block
+    variants pair blocks (0, 1), (2, 3) and leave a last odd block alone;
+    every variant matches the oracle's pair counts and energies."""
+    c = GC.case(20 + nl, 600, nl, 6, 4, 3, 8)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, pairs0 = pr.oracle_scores(poses)
+    assert pairs0.sum() > 0
+    denom = GC.sum_abs_terms(pr, poses)
+    e0 = (4.7 - s0) / 0.0157
+    for v in VARIANTS:
+        s, pairs = score32(pr.gpu, poses, v)
+        assert np.array_equal(pairs, pairs0), v
+        e = (4.7 - s) / 0.0157
+        assert np.all(np.abs(e - e0) <= REL_E * np.maximum(denom, 1.0)), v
+
+
 def test_bins_exact_on_lattice_boundaries():
     """Lattice geometry puts many pairs exactly on k^2/4 boundaries, on the
     225 A^2 cutoff and at d2 = 0: both forms must bin them as the oracle."""
