@@ -173,7 +173,7 @@ struct sdgso_complex_s {
     DevBuf<double> r0, rmodes, l0, lmodes;
     DevBuf<int> rtype, ltype;
     DevBuf<float> table;
-    DevBuf<float4> tiles, lblocks;
+    DevBuf<float4> tiles, lblocks, lhalf;
     DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
     DevBuf<float4> rpos32;     // [ntiles*64] f32 coordinates (near test)
@@ -332,7 +332,7 @@ static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<
     std::vector<double> v(xyz, xyz + 3 * (size_t)n);
     std::vector<int> order = block_order(v, n);
     const int nb = (n + kBlock - 1) / kBlock;
-    std::vector<float4> blocks(nb);
+    std::vector<float4> blocks(nb), half(nb);
     std::vector<int> perm((size_t)nb * kBlock, -1);
     for (int b = 0; b < nb; b++) {
         const int a0 = b * kBlock, a1 = std::min(n, a0 + kBlock);
@@ -342,8 +342,14 @@ static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<
                 lo[k] = std::min(lo[k], xyz[3 * (size_t)order[s] + k]);
                 hi[k] = std::max(hi[k], xyz[3 * (size_t)order[s] + k]);
             }
-        float cf[3];
-        for (int k = 0; k < 3; k++) cf[k] = (float)(0.5 * (lo[k] + hi[k]));
+        float cf[3], hf[3];
+        for (int k = 0; k < 3; k++) {
+            cf[k] = (float)(0.5 * (lo[k] + hi[k]));
+            // the box about the (float) centre, rounded outwards
+            const double h = std::max(hi[k] - (double)cf[k], (double)cf[k] - lo[k]);
+            hf[k] = (float)(h * (1 + 1e-6) + 1e-4);
+        }
+        half[b] = make_float4(hf[0], hf[1], hf[2], 0.f);
         double rad = 0;
         for (int s = a0; s < a1; s++) {
             double d2 = 0;
@@ -381,6 +387,7 @@ static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<
                 B[(size_t)(3 + 3 * m + k) * kBlock] = msoa[((size_t)m * 3 + k) * n + i];
     }
     c->lblocks.upload(blocks);
+    c->lhalf.upload(half);
     c->lperm.upload(perm);
     c->lblk.upload(lb);
     c->dev.nlblocks = nb;
@@ -469,6 +476,7 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.rperm = c->rperm.p;
         d.lperm = c->lperm.p;
         d.lblocks = c->lblocks.p;
+        d.lhalf = c->lhalf.p;
         d.lblk = c->lblk.p;
         d.rrel = c->rrel.p;
         *out = c.release();

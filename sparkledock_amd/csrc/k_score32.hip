@@ -135,6 +135,7 @@ struct WaveCtx {
     float fc[3];
     float reach;      // cutoff + block radius + displacement bounds
     float dlr;        // bound of the mode displacements dl + dr (A)
+    float dl;         // bound of the ligand's mode displacements (A)
     float band;       // this wave's guard band (A^2): the run's, widened to the error bound
     Rot64 rot;        // the pose's rotation matrix (set-up only)
     float rmax;       // bound of |r| for the receptor atoms that pass the atom test
@@ -228,6 +229,7 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     // the FP32 sum of squares, |d2 - d2_ref| <= 9.04e-5 + 3.098e-6 k S A^2.
     x.band = fmaxf(a.band, 9.1e-5f + 3.11e-6f * (float)max(11, c.nmr + 2) * dr);
     x.dlr = dl + dr;
+    x.dl = dl;
     x.reach = 15.0f + blk.w + dl + dr + 0.01f;
     x.rmax = x.reach + dr + 0.01f;
     x.fc[0] = (float)x.cb[0];
@@ -732,6 +734,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     __shared__ int s_q[kWpb][kQueue];        // queued pairs (LDS offsets of both atoms)
     __shared__ float s_ef[kWpb][SDGSO_MAX_MODES];
     __shared__ unsigned short s_ct[kWpb][kCandB];  // candidate tiles, scanned in chunks
+    __shared__ float4 s_box[kWpb][5];        // the box test's parameters (read once per batch)
     const int w = kWpb == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int wv = blockIdx.x * kWpb + w;
     if (wv >= a.n_work) return;
@@ -793,6 +796,32 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         }
         lg[64 + lane] = LB;
         x.ncand = scan_candidates<true>(ntiles, tiles, x, ct, x.t_scan, kCandB, fb, reach_b);
+    }
+    // Box test of a packed atom r (about block b's centre): u = R^T r is the
+    // atom in the ligand frame about that centre, and a pair with an atom of
+    // a block can count only if u lies within 15 A + the ligand displacement
+    // bound (+ 0.01 A for the FP32 rounding of u, at most 1e-4 A) of the
+    // block's box.  Block b + 1's box is offset by the difference of centres.
+    // Kept in LDS: the rows of R^T, (15 + dl + 0.01)^2, the offset of the
+    // centres and both boxes' half extents (-1e30 without a second block, a
+    // box that no atom is near).
+    float4 *box = s_box[w];
+    if (lane == 0) {
+        const float4 hA = gld(c.lhalf, b);
+        const float box_lim = 15.0f + x.dl + 0.01f;
+        float4 hB = make_float4(-1e30f, -1e30f, -1e30f, 0.f);
+        float4 off = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (kLB == 2 && b2 >= 0) {
+            hB = gld(c.lhalf, b2);
+            const float4 ca = gld(c.lblocks, b), cbb = gld(c.lblocks, b2);
+            off = make_float4(ca.x - cbb.x, ca.y - cbb.y, ca.z - cbb.z, 0.f);
+        }
+        const double *m = x.rot.m;
+        box[0] = make_float4((float)m[0], (float)m[3], (float)m[6], box_lim * box_lim);
+        box[1] = make_float4((float)m[1], (float)m[4], (float)m[7], off.y);
+        box[2] = make_float4((float)m[2], (float)m[5], (float)m[8], off.z);
+        box[3] = make_float4(hA.x, hA.y, hA.z, off.x);
+        box[4] = make_float4(hB.x, hB.y, hB.z, 0.f);
     }
     // the set-up's wave-uniform results, in scalar registers
     const double cb0 = uniform_f64(x.cb[0]), cb1 = uniform_f64(x.cb[1]), cb2 = uniform_f64(x.cb[2]);
@@ -1002,28 +1031,56 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             wave_sync_lds();
             if (lane < n) D = rf[lane];
         }
-        // packing position: the lane's own, or with two blocks, the batch
-        // ordered as [near block b only | near both | near block b + 1 only], so
-        // that each block's atoms form one range, [0, a_end) and [b_begin, n)
-        int pos = lane, a_end = n, b_begin = 0;
-        if constexpr (kLB == 2) {
-            const unsigned long long m_a = __ballot(lane < n && fl == 1u);
-            const unsigned long long m_ab = __ballot(lane < n && fl == 3u);
-            const unsigned long long m_b = __ballot(lane < n && fl == 2u);
-            const int n_a = __popcll(m_a), n_ab = __popcll(m_ab);
-            const unsigned long long mine = fl == 1u ? m_a : (fl == 3u ? m_ab : m_b);
-            pos = (fl == 1u ? 0 : (fl == 3u ? n_a : n_a + n_ab)) +
-                  (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32),
-                                                 __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
-            a_end = n_a + n_ab;
-            b_begin = n_a;
-            wave_sync_lds();  // every lane has read its staged displacement
-        }
+        // this lane's atom on the block's grid, and in FP32
+        int4 R = make_int4(kFar, kFar, kFar, 0);
+        float fx = 0.f, fy = 0.f, fz = 0.f;
         if (lane < n) {
-            const int4 R = make_int4(qround(((double)tc.x - cb0) + rxy.x + (double)D.x),
-                                     qround(((double)tc.y - cb1) + rxy.y + (double)D.y),
-                                     qround(((double)tc.z - cb2) + rz + (double)D.z), row);
-            const float fx = (float)R.x * kInvQ, fy = (float)R.y * kInvQ, fz = (float)R.z * kInvQ;
+            R = make_int4(qround(((double)tc.x - cb0) + rxy.x + (double)D.x),
+                          qround(((double)tc.y - cb1) + rxy.y + (double)D.y),
+                          qround(((double)tc.z - cb2) + rz + (double)D.z), row);
+            fx = (float)R.x * kInvQ;
+            fy = (float)R.y * kInvQ;
+            fz = (float)R.z * kInvQ;
+        }
+        // box test (see the set-up): the near flags of atoms that cannot
+        // count with a block are cleared
+        {
+            const float4 q0 = box[0], q1 = box[1], q2 = box[2], hA = box[3];
+            const float ux = __builtin_fmaf(q0.x, fx, __builtin_fmaf(q0.y, fy, q0.z * fz));
+            const float uy = __builtin_fmaf(q1.x, fx, __builtin_fmaf(q1.y, fy, q1.z * fz));
+            const float uz = __builtin_fmaf(q2.x, fx, __builtin_fmaf(q2.y, fy, q2.z * fz));
+            auto in_box = [&](float vx, float vy, float vz, float4 h) {
+                const float ex = fmaxf(fabsf(vx) - h.x, 0.f);
+                const float ey = fmaxf(fabsf(vy) - h.y, 0.f);
+                const float ez = fmaxf(fabsf(vz) - h.z, 0.f);
+                return __builtin_fmaf(ez, ez, __builtin_fmaf(ey, ey, ex * ex)) <= q0.w;
+            };
+            unsigned keep = in_box(ux, uy, uz, hA) ? 1u : 0u;
+            if constexpr (kLB == 2) {
+                const float4 hB = box[4];
+                if (in_box(ux + hA.w, uy + q1.w, uz + q2.w, hB)) keep |= 2u;
+                fl &= keep;
+            } else {
+                fl = keep;
+            }
+            if (lane >= n) fl = 0u;
+        }
+        // packing position: the atoms near the block first, [0, a_end), then
+        // the others; with two blocks the order is [near block b only | near
+        // both | near block b + 1 only | near neither], so that each block's
+        // atoms form one range, [0, a_end) and [b_begin, b_end)
+        const unsigned long long m_a = __ballot(fl == 1u);
+        const unsigned long long m_ab = __ballot(fl == 3u);
+        const unsigned long long m_b = __ballot(fl == 2u);
+        const unsigned long long m_0 = __ballot(lane < n && fl == 0u);
+        const int n_a = __popcll(m_a), n_ab = __popcll(m_ab), n_b = __popcll(m_b);
+        const unsigned long long mine = fl == 1u ? m_a : (fl == 3u ? m_ab : (fl == 2u ? m_b : m_0));
+        const int pos = (fl == 1u ? 0 : (fl == 3u ? n_a : (fl == 2u ? n_a + n_ab : n_a + n_ab + n_b))) +
+                        (int)__builtin_amdgcn_mbcnt_hi((unsigned)(mine >> 32),
+                                                       __builtin_amdgcn_mbcnt_lo((unsigned)mine, 0u));
+        const int a_end = n_a + n_ab, b_begin = n_a, b_end = n_a + n_ab + n_b;
+        wave_sync_lds();  // every lane has read its staged displacement
+        if (lane < n) {
             rt[64 * p + pos] = R;
             rf[pos] = make_float4(fx, fy, fz, __builtin_fmaf(fz, fz, __builtin_fmaf(fy, fy, fx * fx)));
             rs[64 * p + pos] = (unsigned short)slot;
@@ -1034,11 +1091,11 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         const unsigned hib = rt0 + 64u * 16u * (unsigned)p;  // offset of this buffer's atom 0
         if (mfma_ok) {
             // lane l: row 8*(v/4) + 4*(l/32) + v%4 of accumulator v, column l%32
-            for (int h = 0; h * 32 < n; h++) {
+            for (int h = 0; h * 32 < a_end; h++) {
                 const int ri = 32 * h + (lane & 31);
                 const float4 r = rf[ri];
                 half4 av;
-                if (ri < n) {
+                if (ri < a_end) {
                     const _Float16 nh = (_Float16)r.w;
                     if (lane < 32)
                         av = half4{(_Float16)r.x, (_Float16)r.y, (_Float16)r.z, nh};
@@ -1111,10 +1168,10 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 }
             };
             pretest(0, a_end, m2x, m2y, m2z, tl, lane_lo);
-            pretest(b_begin & ~3, n, m2xb, m2yb, m2zb, tlb, lane_lo + 64u * 16u);
+            pretest(b_begin & ~3, b_end, m2xb, m2yb, m2zb, tlb, lane_lo + 64u * 16u);
         } else {
             // four packed atoms per round: their broadcast reads are in flight together
-            for (int i0 = 0; i0 < n; i0 += 4) {
+            for (int i0 = 0; i0 < a_end; i0 += 4) {
                 float4 r[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) r[j] = rf[min(i0 + j, 63)];
@@ -1125,7 +1182,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                     const float wv2 = __builtin_fmaf(
                         r[j].x, m2x, __builtin_fmaf(r[j].y, m2y, __builtin_fmaf(r[j].z, m2z, r[j].w)));
                     in[j] = wv2 <= tl;
-                    m[j] = i0 + j < n ? __ballot(in[j]) : 0ull;
+                    m[j] = i0 + j < a_end ? __ballot(in[j]) : 0ull;
                 }
 #pragma unroll
                 for (int j = 0; j < 4; j++) {

@@ -18,6 +18,7 @@
 //   ligand blocks (FP32 guard form): 64 atoms per block, grouped by bisection
 //     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
 //     lblocks f32x4 [nlblocks]     block centre in the ligand frame, radius
+//     lhalf   f32x4 [nlblocks]     half extents of the block's box about that centre
 //     lblk    f64 [nlblocks][3+3*nml][64]  coordinates and modes of the block slots
 //   run state, N = sum(swarms) * G glowworms
 //     pos [N][P] f64, luciferin/score/vision [N] f64, ref [N][3] f64,
@@ -79,6 +80,7 @@ struct DevComplex {
     int nlblocks;
     const int *lperm;
     const float4 *lblocks;
+    const float4 *lhalf;     // [nlblocks] half extents of the block's box (ligand frame)
     const double *lblk;      // [nlblocks][3 + 3*nml][64] coordinates, then modes, by block slot
     float lmode_max[SDGSO_MAX_MODES];
 };
