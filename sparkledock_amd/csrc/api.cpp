@@ -181,7 +181,8 @@ struct sdgso_complex_s {
     DevBuf<float> rmodA;       // [ntiles*64][rmod_stride] modes by slot
     int rmod_stride = 0;
     DevBuf<int> rtype_s, rperm, lperm;
-    DevBuf<double> lblk;       // [nlblocks][3 + 3*nml][64] ligand coordinates and modes by block slot
+    DevBuf<double> lblk;       // [nlblocks][3][64] ligand coordinates by block slot
+    DevBuf<float> lblkm;       // [nlblocks][3*nml][64] ligand modes by block slot, FP32
     DevComplex dev{};          // descriptor with device pointers
     double max_rec_extent = 0; // |xyz| maxima, for the guard-band bound
 };
@@ -374,22 +375,25 @@ static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<
         }
         c->dev.lmode_max[m] = (float)(mx * (1 + 1e-5));
     }
-    // coordinates and modes of every block slot, slot-contiguous (padding: zeros)
-    const int rows = 3 + 3 * nm;
-    std::vector<double> lb((size_t)nb * rows * kBlock, 0.0);
+    // coordinates (FP64) and modes (FP32) of every block slot, slot-contiguous
+    // (padding: zeros)
+    std::vector<double> lb((size_t)nb * 3 * kBlock, 0.0);
+    std::vector<float> lbm((size_t)nb * 3 * nm * kBlock, 0.0f);
     for (int s = 0; s < nb * kBlock; s++) {
         const int i = perm[s];
         if (i < 0) continue;
-        double *B = lb.data() + (size_t)(s / kBlock) * rows * kBlock + s % kBlock;
+        double *B = lb.data() + (size_t)(s / kBlock) * 3 * kBlock + s % kBlock;
+        float *BM = lbm.data() + (size_t)(s / kBlock) * 3 * nm * kBlock + s % kBlock;
         for (int k = 0; k < 3; k++) B[(size_t)k * kBlock] = xyz[3 * (size_t)i + k];
         for (int m = 0; m < nm; m++)
             for (int k = 0; k < 3; k++)
-                B[(size_t)(3 + 3 * m + k) * kBlock] = msoa[((size_t)m * 3 + k) * n + i];
+                BM[(size_t)(3 * m + k) * kBlock] = (float)msoa[((size_t)m * 3 + k) * n + i];
     }
     c->lblocks.upload(blocks);
     c->lhalf.upload(half);
     c->lperm.upload(perm);
     c->lblk.upload(lb);
+    c->lblkm.upload(lbm);
     c->dev.nlblocks = nb;
 }
 
@@ -478,6 +482,7 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.lblocks = c->lblocks.p;
         d.lhalf = c->lhalf.p;
         d.lblk = c->lblk.p;
+        d.lblkm = c->lblkm.p;
         d.rrel = c->rrel.p;
         *out = c.release();
     });

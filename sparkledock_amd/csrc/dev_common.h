@@ -151,43 +151,49 @@ __device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double 
 
 // The same for slot `lane` of ligand block b, read from the block-slot copy
 // (every load of a wave is contiguous), for a position that the FP32 kernels
-// round to their grid: the modes are added with FMAs and the rotation is the
-// matrix R of the pose, so the result is within a few FP64 ulps of
-// ligand_atom64's.
+// round to their grid: the mode displacement is an FP32 FMA chain in mode
+// order over FP32 modes and extents (within (nml + 2) u S of the exact one,
+// u = 2^-24 and S the pose's ligand displacement bound), added to the FP64
+// coordinates, which the pose's matrix R then rotates.
 __device__ __forceinline__ void ligand_slot_grid(const DevComplex &c, const double *pose,
                                                  const Rot64 &R, int b, int lane, double &x,
                                                  double &y, double &z) {
     const int nm = c.nml;
-    const double *B = c.lblk + (size_t)b * (3 + 3 * nm) * kBlock + lane;
+    const double *B = c.lblk + (size_t)b * 3 * kBlock + lane;
+    const float *BM = c.lblkm + (size_t)b * 3 * nm * kBlock + lane;
     x = gld(B, 0);
     y = gld(B, kBlock);
     z = gld(B, 2 * kBlock);
     const double *ext = pose + 7 + c.nmr;
+    float dx = 0.f, dy = 0.f, dz = 0.f;
     int m = 0;
     for (; m + 4 <= nm; m += 4) {
-        double mx[4], my[4], mz[4], e[4];
+        float mx[4], my[4], mz[4], e[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const double *M = B + (size_t)(3 + 3 * (m + k)) * kBlock;
+            const float *M = BM + (size_t)(3 * (m + k)) * kBlock;
             mx[k] = gld(M, 0);
             my[k] = gld(M, kBlock);
             mz[k] = gld(M, 2 * kBlock);
-            e[k] = ext[m + k];
+            e[k] = (float)ext[m + k];
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            x = fma(mx[k], e[k], x);
-            y = fma(my[k], e[k], y);
-            z = fma(mz[k], e[k], z);
+            dx = __builtin_fmaf(mx[k], e[k], dx);
+            dy = __builtin_fmaf(my[k], e[k], dy);
+            dz = __builtin_fmaf(mz[k], e[k], dz);
         }
     }
     for (; m < nm; m++) {
-        const double *M = B + (size_t)(3 + 3 * m) * kBlock;
-        const double e = ext[m];
-        x = fma(gld(M, 0), e, x);
-        y = fma(gld(M, kBlock), e, y);
-        z = fma(gld(M, 2 * kBlock), e, z);
+        const float *M = BM + (size_t)(3 * m) * kBlock;
+        const float e = (float)ext[m];
+        dx = __builtin_fmaf(gld(M, 0), e, dx);
+        dy = __builtin_fmaf(gld(M, kBlock), e, dy);
+        dz = __builtin_fmaf(gld(M, 2 * kBlock), e, dz);
     }
+    x += (double)dx;
+    y += (double)dy;
+    z += (double)dz;
     rot_apply(R, x, y, z);
     x += pose[0];
     y += pose[1];

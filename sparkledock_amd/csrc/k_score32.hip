@@ -222,12 +222,14 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     // Guard band that keeps this wave's bins exact.  Per axis the fixed-point
     // coordinates differ from the reference's FP64 ones by at most 1.01 grid
     // units (two roundings to 2^-20 A, FP64 noise) plus the FP32 displacement
-    // error k u S, u = 2^-24, S <= dr the displacement bound and k the
-    // roundings per mode term (FP32 mode and extent, and the sum: at most 11 for
-    // the broadcast variant's in-lane FMAs and group sum, nmr + 2 for a chain
-    // in mode order).  With |dx| + |dy| + |dz| <= 15 sqrt(3) and 3 u 225 for
-    // the FP32 sum of squares, |d2 - d2_ref| <= 9.04e-5 + 3.098e-6 k S A^2.
-    x.band = fmaxf(a.band, 9.1e-5f + 3.11e-6f * (float)max(11, c.nmr + 2) * dr);
+    // errors k u S of both atoms, u = 2^-24, S the displacement bound (dr,
+    // dl) and k the roundings per mode term (FP32 mode and extent, and the
+    // sum: at most 11 for the broadcast variant's in-lane FMAs and group sum,
+    // nm + 2 for a chain in mode order, as the ligand's).  With |dx| + |dy| +
+    // |dz| <= 15 sqrt(3) and 3 u 225 for the FP32 sum of squares,
+    // |d2 - d2_ref| <= 9.04e-5 + 3.098e-6 (k_r dr + k_l dl) A^2.
+    x.band = fmaxf(a.band, 9.1e-5f + 3.11e-6f * ((float)max(11, c.nmr + 2) * dr +
+                                                   (float)(c.nml + 2) * dl));
     x.dlr = dl + dr;
     x.dl = dl;
     x.reach = 15.0f + blk.w + dl + dr + 0.01f;

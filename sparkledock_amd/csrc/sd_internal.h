@@ -19,7 +19,8 @@
 //     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
 //     lblocks f32x4 [nlblocks]     block centre in the ligand frame, radius
 //     lhalf   f32x4 [nlblocks]     half extents of the block's box about that centre
-//     lblk    f64 [nlblocks][3+3*nml][64]  coordinates and modes of the block slots
+//     lblk    f64 [nlblocks][3][64]        coordinates of the block slots
+//     lblkm   f32 [nlblocks][3*nml][64]    modes of the block slots (FP32)
 //   run state, N = sum(swarms) * G glowworms
 //     pos [N][P] f64, luciferin/score/vision [N] f64, ref [N][3] f64,
 //     nnei/moved/select/step [N] i32, pairs [N] i64,
@@ -81,7 +82,8 @@ struct DevComplex {
     const int *lperm;
     const float4 *lblocks;
     const float4 *lhalf;     // [nlblocks] half extents of the block's box (ligand frame)
-    const double *lblk;      // [nlblocks][3 + 3*nml][64] coordinates, then modes, by block slot
+    const double *lblk;      // [nlblocks][3][64] coordinates by block slot
+    const float *lblkm;      // [nlblocks][3*nml][64] modes by block slot, FP32
     float lmode_max[SDGSO_MAX_MODES];
 };
 

@@ -233,6 +233,53 @@ def test_bins_exact_with_large_displacements():
         assert np.array_equal(per_bin_counts(pr, poses, None, v)[:, 0], want), v
 
 
+def large_ligand_displacement_case(seed=21, nm=10, ext=25.0):
+    """The ligand side of the case above: ten ligand modes with extents of
+    tens of A move every ligand atom onto d2 = k^2/4 + delta from a receptor
+    atom at the origin (the FP32 ligand mode sum is off by a few 1e-6 A)."""
+    rng = np.random.default_rng(seed)
+    deltas = np.concatenate([np.linspace(-3e-3, 3e-3, 21), np.linspace(-4e-4, 4e-4, 81)])
+    d2 = np.array([k * k / 4.0 + d for k in range(2, 31) for d in deltas])
+    n = len(d2)
+    l_modes = rng.normal(size=(nm, n, 3)) * 0.6
+    e = rng.uniform(-ext, ext, nm)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    target = dirs * np.sqrt(d2)[:, None]
+    D = np.zeros((n, 3))
+    for m in range(nm):  # the reference's order: product rounded, then added
+        D = D + l_modes[m] * e[m]
+    c = dict(r_xyz=np.array([[0.0, 0.0, 0.0], [400.0, 0.0, 0.0]]),
+             l_xyz=target - D,
+             r_type=np.array([5, 9], np.int32), l_type=rng.integers(0, 167, n).astype(np.int32),
+             table=(rng.normal(size=168 * 168 * 20) * 0.5).astype(np.float32),
+             r_modes=np.zeros((0, 2, 3)), l_modes=l_modes,
+             r_mask=np.ones(2, np.int32), l_mask=np.ones(n, np.int32),
+             lig_ref=np.zeros(3))
+    c["pos"] = np.concatenate([[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0], e])[None, None, :]
+    return c
+
+
+def test_bins_exact_with_large_ligand_displacements():
+    """The band also covers the FP32 ligand mode sum: bins stay exact at every
+    boundary for ligand atoms that large ligand modes displace."""
+    c = large_ligand_displacement_case()
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, p0 = pr.oracle_scores(poses)
+    want = []
+    for b in range(20):
+        t = np.zeros((168, 168, 20), np.float32)
+        t[:, :, b] = 1.0
+        want.append(round((4.7 - pr.with_table(t.reshape(-1)).score_pose(poses[0])[0]) / 0.0157))
+    want = np.array(want)
+    assert want.sum() == p0[0] > 1000
+    for v in VARIANTS:
+        s, p = score32(pr.gpu, poses, v)
+        assert np.array_equal(p, p0) and abs(s[0] - s0[0]) < 1e-5, v
+        assert np.array_equal(per_bin_counts(pr, poses, None, v)[:, 0], want), v
+
+
 def test_bins_exact_random_fp32():
     c = GC.case(8, 900, 400, 10, 10, 3, 8)
     pr = GC.Pair(c)
