@@ -177,7 +177,7 @@ struct sdgso_complex_s {
     DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
     DevBuf<float4> rpos32;     // [ntiles*64] f32 coordinates (near test)
-    DevBuf<double> rrel4;      // [ntiles*64][4] f64 coordinates - tile centre
+    DevBuf<double> rrel4;      // [ntiles*64][4] f64 absolute coordinates, type row
     DevBuf<float> rmodA;       // [ntiles*64][rmod_stride] modes by slot
     int rmod_stride = 0;
     DevBuf<int> rtype_s, rperm, lperm;
@@ -298,10 +298,14 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
                 // padding slot: far away, type 0, zero modes
                 for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = 1.0e4;
             }
-            for (int k = 0; k < 3; k++) r4[4 * (size_t)s + k] = rrel[(size_t)k * slots + s];
-            p32[s] = make_float4((float)(r4[4 * (size_t)s] + (double)cf[0]),
-                                 (float)(r4[4 * (size_t)s + 1] + (double)cf[1]),
-                                 (float)(r4[4 * (size_t)s + 2] + (double)cf[2]), 0.0f);
+            // absolute coordinates (the input's, or far away for padding) and
+            // the type row, exact as a double
+            for (int k = 0; k < 3; k++)
+                r4[4 * (size_t)s + k] = s < a1 ? xyz[3 * (size_t)order[s] + k]
+                                               : rrel[(size_t)k * slots + s] + (double)cf[k];
+            r4[4 * (size_t)s + 3] = (double)ts[s];
+            p32[s] = make_float4((float)r4[4 * (size_t)s], (float)r4[4 * (size_t)s + 1],
+                                 (float)r4[4 * (size_t)s + 2], 0.0f);
         }
     }
     for (int m = 0; m < nm; m++) {

@@ -750,7 +750,6 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     const float4 *rpos32 = uniform_ptr(c.rpos32);
     const double *rrel4 = uniform_ptr(c.rrel4);
     const float4 *rmodA = uniform_ptr((const float4 *)c.rmodA);
-    const int *rtype_s = uniform_ptr(c.rtype_s);
     const __amdgpu_buffer_rsrc_t tbl = __builtin_amdgcn_make_buffer_rsrc(
         (void *)uniform_ptr(c.table), (short)0, kTableLen * 4, 0x00020000);
     int4 *rt = s_rt[w];
@@ -964,21 +963,17 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     // queued at the end are evaluated too, since the next batch overwrites it.
     auto batch = [&](int n) {
         old = qn;
-        // this lane's atom: slot, near flags, tile centre, coordinates and type
-        // row (loads first)
-        int slot = 0, row = 0;
+        // this lane's atom: slot, near flags, FP64 coordinates and type row
+        // (loads first)
+        int slot = 0;
         unsigned fl = 0;
-        float4 tc = make_float4(0.f, 0.f, 0.f, 0.f);
-        double2 rxy = make_double2(0.0, 0.0);
-        double rz = 0.0;
+        double2 rxy = make_double2(0.0, 0.0), rzw = make_double2(0.0, 0.0);
         if (lane < n) {
             const unsigned e = pd[lane];
             slot = (int)(e & 0xffffu);
             fl = e >> 16;
-            tc = gld(tiles, slot / kTile);
             rxy = gld2(rrel4, 2 * (size_t)slot);
-            rz = gld(rrel4, 4 * (size_t)slot + 2);
-            row = gld(rtype_s, slot);
+            rzw = gld2(rrel4, 2 * (size_t)slot + 1);
         }
         // displacement D = sum_m M_m e_m of the batch's atoms, kC lanes per atom: in
         // round k lane l takes float4 j = l % kC of atom k * 64 / kC + l / kC (an
@@ -1037,9 +1032,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         int4 R = make_int4(kFar, kFar, kFar, 0);
         float fx = 0.f, fy = 0.f, fz = 0.f;
         if (lane < n) {
-            R = make_int4(qround(((double)tc.x - cb0) + rxy.x + (double)D.x),
-                          qround(((double)tc.y - cb1) + rxy.y + (double)D.y),
-                          qround(((double)tc.z - cb2) + rz + (double)D.z), row);
+            R = make_int4(qround((rxy.x - cb0) + (double)D.x), qround((rxy.y - cb1) + (double)D.y),
+                          qround((rzw.x - cb2) + (double)D.z), (int)rzw.y);
             fx = (float)R.x * kInvQ;
             fy = (float)R.y * kInvQ;
             fz = (float)R.z * kInvQ;

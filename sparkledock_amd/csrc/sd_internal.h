@@ -68,8 +68,9 @@ struct DevComplex {
     const double *rrel;      // [3][ntiles*64] coordinates - tile centre (f64)
     const float *rmod32;     // [nmr][3][ntiles*64] modes (f32)
     // the same per slot, for gathers by slot: near-test coordinates (f32,
-    // absolute, w = 0), coordinates - tile centre (f64 x, y, z, 0) and modes
-    // (f32, mode-major x, y, z, padded to 4 * mode_chunks(nmr) = rmod_stride floats)
+    // absolute, w = 0), coordinates and type row (f64 absolute x, y, z and
+    // the row rtype_s as a double) and modes (f32, mode-major x, y, z,
+    // padded to 4 * mode_chunks(nmr) = rmod_stride floats)
     const float4 *rpos32;
     const double *rrel4;
     const float *rmodA;
