@@ -1,43 +1,46 @@
 // FP32 guard-band scoring (the batched form).
 //
-// Work unit: one wavefront per (glowworm, ligand block of 64 atoms), taken
+// Work unit: one wavefront per (glowworm, ligand block of 64 atoms), or per
+// (glowworm, pair of neighbouring blocks) in the two-block variants, taken
 // from a work list that keeps the blocks of one glowworm adjacent.  Lane l
-// owns ligand atom lperm[block*64 + l], transformed in FP64 in the reference
-// order (prepare_gso_tasks.cc:487-526) and rounded once to a 2^-20 A
-// fixed-point grid whose origin is the block centre.
+// owns ligand atom lperm[block*64 + l]: its FP64 coordinates plus an FP32
+// mode sum, rotated by the pose's FP64 matrix and translated
+// (prepare_gso_tasks.cc:487-526 up to rounding), then rounded once to a
+// 2^-20 A fixed-point grid whose origin is the (first) block's centre.
 //
 // Receptor tiles: compact groups of 64 atoms (recursive bisection) with
 // bounding spheres.  The wave tests 64 tile spheres per instruction against
 // the block sphere, padded by the cutoff and this pose's normal-mode
-// displacement bounds, and lists the candidates in LDS; it visits them in
-// order while the next tile's coordinates load.  In a visited tile each lane
-// displaces one receptor atom (FP32 modes, loads batched four modes deep),
-// rounds it to the same grid and the wave keeps the atoms that can reach the
-// block (ballot).
+// displacement bounds, lists the candidates in LDS and visits them; the
+// receptor atoms that can reach the block are displaced (FP32 modes), rounded
+// to the same grid and pre-tested against the block's ligand atoms.
 //
-// Filter (two variants, chosen by the run's autotuner):
-//   readlane: the kept receptor atoms are broadcast one at a time; every lane
-//     forms the exact fixed-point d2 with its ligand atom and in-cutoff pairs
-//     are appended to a per-wave LDS queue (d2, table row, tag);
-//   mfma: the kept atoms are packed into LDS and d2 = |r|^2 + |l|^2 - 2 r.l of
-//     32 x 32 pairs is three v_mfma_f32_32x32x2f32 (r' = (x, y, z, |r|^2, 1, 0),
-//     l' = (-2x, -2y, -2z, 1, |l|^2, 0)); its FP32 cancellation error is below
-//     2e-3 A^2 in the block frame, so pairs with d2 <= 225 + 2 band + 0.05 are
-//     queued (receptor, ligand) and evaluated exactly.
+// Kernels (the run's autotuner picks one per complex):
+//   score32b ("bcast", "bcast2"): near atoms are batched, displaced by
+//     cooperative lanes, box-tested, packed into LDS and pre-tested with
+//     broadcast reads (FP32 or half-precision MFMA); passing pairs are queued
+//     as one word and evaluated 64 per round (described at the kernel);
+//   score32r ("readlane"): kept receptor atoms are broadcast one at a time
+//     and every lane forms the exact fixed-point d2 with its ligand atom;
+//   score32m ("mfma"): d2 = |r|^2 + |l|^2 - 2 r.l of 32 x 32 pairs by three
+//     v_mfma_f32_32x32x2f32 (r' = (x, y, z, |r|^2, 1, 0), l' = (-2x, -2y, -2z,
+//     1, |l|^2, 0)); its FP32 cancellation error is below 2e-3 A^2 in the
+//     block frame, so pairs with d2 <= 225 + 2 band + 0.05 are queued and
+//     evaluated exactly.
 //
 // Evaluate (all 64 lanes busy): exact d2 from the fixed-point coordinates
 // (integer differences are exact and convert to FP32 exactly below 16 A), the
 // bin, and the table value; several gathers are in flight per lane before the
 // compensated (Kahan) FP32 adds.  The FP32 d2 differs from the FP64 reference
-// d2 by at most 9.04e-5 + 3.1e-6 k S A^2 (S the pose's receptor displacement
-// bound, k the FP32 roundings per mode term): 9.7e-5 A^2 for the synthetic
-// workloads, under the default 1e-4 A^2 band, which a wave widens to the bound
-// when the bound is larger.  A pair within the band of a boundary k^2/4
-// (k = 0..30: d2 = 0, every bin edge and the 225 A^2 cutoff) is recomputed in
-// FP64 from the reference-order coordinates, so every bin and in/out decision
-// equals the reference's.  Outside every band a pair
-// is inside iff d2 < 225 and its bin is floor(2 sqrt(d2)) through dist_to_bins
-// (raw v_sqrt_f32, 1 ulp, cannot cross an integer there).
+// d2 by at most 9.04e-5 + 3.1e-6 (k_r S_r + k_l S_l) A^2 (S the pose's
+// receptor and ligand displacement bounds, k the FP32 roundings per mode
+// term); each wave widens the run's band (default 1e-4 A^2) to that bound.  A
+// pair within the band of a boundary k^2/4 (k = 0..30: d2 = 0, every bin edge
+// and the 225 A^2 cutoff) is recomputed in FP64 from the reference-order
+// coordinates, so every bin and in/out decision equals the reference's.
+// Outside every band a pair is inside iff d2 < 225 and its bin is
+// floor(2 sqrt(d2)) through dist_to_bins (raw v_sqrt_f32, 1 ulp, cannot cross
+// an integer there).
 //
 // Lanes are combined in FP64 and written as one partial per (glowworm,
 // block); finish32 adds a glowworm's partials in block order in FP64.
