@@ -1,6 +1,8 @@
 // Device helpers that restate the reference's FP64 pose arithmetic in its
-// operation order.  Every translation unit that includes this file is built
-// with -ffp-contract=off, so no multiply-add here is fused.
+// operation order, and the FP32 form's grid-position helpers (rotation matrix,
+// FP32 mode sums) that are close to it within stated bounds.  Every
+// translation unit that includes this file is built with -ffp-contract=off,
+// so no multiply-add is fused unless it is written as an fma.
 #pragma once
 
 #include <hip/hip_runtime.h>
