@@ -109,7 +109,8 @@ struct ScoreArgs {
     long long *partial_pairs;  // [n_glow][max_lblocks] per-block pair counts
     int max_lblocks;
     int nmr;                // receptor modes of every complex of the launch
-    const int *work;        // FP32 form: work list of (glowworm, ligand block), or null
+    const int *work;        // FP32 form: (glowworm, ligand block) pairs, or (glowworm, first
+                            // block of a pair) for two-block variants; null otherwise
     int n_work;
     // FP32 form, optional: filter statistics [6][kStatSlots] (waves, waves
     // with candidate tiles, candidate tiles, near receptor atoms, queued
