@@ -31,7 +31,7 @@
 // Evaluate (all 64 lanes busy): exact d2 from the fixed-point coordinates
 // (integer differences are exact and convert to FP32 exactly below 16 A), the
 // bin, and the table value; several gathers are in flight per lane before the
-// compensated (Kahan) FP32 adds.  The FP32 d2 differs from the FP64 reference
+// compensated FP32 adds (TwoSum per term; sum + error term kept to the end).  The FP32 d2 differs from the FP64 reference
 // d2 by at most 9.04e-5 + 3.1e-6 (k_r S_r + k_l S_l) A^2 (S the pose's
 // receptor and ligand displacement bounds, k the FP32 roundings per mode
 // term); each wave widens the run's band (default 1e-4 A^2) to that bound.  A
@@ -42,8 +42,9 @@
 // floor(2 sqrt(d2)) through dist_to_bins (raw v_sqrt_f32, 1 ulp, cannot cross
 // an integer there).
 //
-// Lanes are combined in FP64 and written as one partial per (glowworm,
-// block); finish32 adds a glowworm's partials in block order in FP64.
+// Lanes (sum + error term) are combined in FP64 and written as one partial per
+// (glowworm, block); finish32 adds a glowworm's partials in block order in
+// FP64, so the energy keeps its relative accuracy when the terms cancel.
 #include <string>
 #include <type_traits>
 
@@ -69,14 +70,19 @@ constexpr float kFilterMargin = 0.05f;        // A^2, over the MFMA filter's rou
 constexpr int kFar = 1 << 29;                 // far-away fixed-point coordinate (512 A)
 constexpr int kZeroIdx = SDGSO_TABLE_LEN + SDGSO_NBINS - 1;  // a zero of the table pad
 
-struct Kahan {
+// Compensated FP32 sum (Neumaier's form of Kahan's method, with Knuth's
+// branch-free TwoSum for each term's rounding error): sum + comp is the exact
+// sum of the terms up to O(n^2 u^2) sum |T|, so a sum whose terms cancel keeps
+// its relative accuracy.  A term not taken is added as 0, which is exact.
+struct CompSum {
     float sum = 0.f, comp = 0.f;
     __device__ __forceinline__ void add(float v, bool take) {
-        const float y = v - comp;
-        const float t = sum + y;
-        const float c2 = (t - sum) - y;
-        sum = take ? t : sum;
-        comp = take ? c2 : comp;
+        v = take ? v : 0.f;
+        const float t = sum + v;
+        const float vb = t - sum;
+        const float err = (sum - (t - vb)) + (v - vb);
+        sum = t;
+        comp += err;
     }
 };
 
@@ -294,11 +300,11 @@ __device__ __forceinline__ bool tile_atom(const DevComplex &c, const WaveCtx &x,
 // The wave's energy and pair count go to block x.b's partial; a second block
 // b2 (>= 0) of a two-block wave gets zeros.
 __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
-                                               const Kahan &acc, int npairs,
+                                               const CompSum &acc, int npairs,
                                                unsigned int nguard, long long n_cand,
                                                long long n_near, long long n_queued,
                                                int b2 = -1) {
-    double e = (double)acc.sum - (double)acc.comp;
+    double e = (double)acc.sum + (double)acc.comp;
     long long np = npairs;
     for (int off = 32; off > 0; off >>= 1) {
         e += __shfl_xor(e, off, 64);
@@ -353,7 +359,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
     const float cutq = (225.0f + 2.0f * x.band) * kQ2;
     const int slots = c.ntiles * kTile;
 
-    Kahan acc;
+    CompSum acc;
     int npairs = 0;
     unsigned int nguard = 0;
     int qn = 0;
@@ -493,7 +499,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
         bop[g][2] = lo_half ? l2 : 0.f;
     }
 
-    Kahan acc;
+    CompSum acc;
     int npairs = 0;
     unsigned int nguard = 0;
     int qn = 0;
@@ -880,7 +886,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         }
     }
 
-    Kahan acc;
+    CompSum acc;
     int npairs = 0;          // in-cutoff pairs evaluated by this lane
     unsigned int nguard = 0;
     int qn = 0;
