@@ -120,6 +120,27 @@ def test_ligand_block_counts(nl):
         assert_energy(e, e0, denom, v)
 
 
+def test_energy_with_terms_of_both_signs_over_eight_decades():
+    """Table values of both signs spread over eight orders of magnitude: the
+    large terms swamp the small ones in a plain or Kahan FP32 sum (errors of
+    about 2^-24 of the running sum).  The compensated sums still hold every
+    variant's energy to the oracle's FP64 sum within 1e-9 |E| + 1e-12 sum |T|."""
+    c = GC.case(31, 700, 300, 6, 4, 3, 16)
+    rng = np.random.default_rng(31)
+    n = len(c["table"])
+    c["table"] = (rng.normal(size=n) * 10.0 ** rng.uniform(-4.0, 4.0, n)).astype(np.float32)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, pairs0 = pr.oracle_scores(poses)
+    assert pairs0.min() > 0
+    denom = GC.sum_abs_terms(pr, poses)
+    e0 = (4.7 - s0) / 0.0157
+    for v in VARIANTS:
+        s, pairs = score32(pr.gpu, poses, v)
+        assert np.array_equal(pairs, pairs0), v
+        assert_energy((4.7 - s) / 0.0157, e0, denom, v)
+
+
 def test_bins_exact_on_lattice_boundaries():
     """Lattice geometry puts many pairs exactly on k^2/4 boundaries, on the
     225 A^2 cutoff and at d2 = 0: both forms must bin them as the oracle."""
