@@ -399,6 +399,31 @@ def test_graph_replay_equals_eager():
             assert np.array_equal(a[key], b[key]), key
 
 
+def test_every_graph_replay_equals_eager():
+    """The captured step checked replay by replay, with host reads and syncs
+    between replays: a run of replays only, and a run that alternates replays
+    with eager steps (both advance the same device-resident step index), each
+    equal after every step to an eager run from the same start."""
+    c = GC.case(21, 400, 200, 10, 10, 3, 32, swarm_radius=3.0)
+    pr = GC.Pair(c)
+    keys = ("pos", "luciferin", "score", "select", "pairs", "vision", "moved", "nnei")
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        eager, _ = make_run(pr, c, 5, form=form)
+        replays, _ = make_run(pr, c, 5, form=form)
+        mixed, _ = make_run(pr, c, 5, form=form)
+        for k in range(5):
+            eager.steps(1)
+            replays.steps(1, graph=True)
+            mixed.steps(1, graph=(k % 2 == 0))
+            want = eager.read()
+            for run in (replays, mixed):
+                got = run.read()
+                for key in keys:
+                    assert np.array_equal(got[key], want[key]), (form, k, key)
+        for run in (eager, replays, mixed):
+            run.close()
+
+
 def test_batch_equals_solo_runs():
     """Two complexes of different sizes in one run give the results of two solo runs."""
     ca = GC.case(12, 350, 180, 10, 10, 2, 32, swarm_radius=3.0)
