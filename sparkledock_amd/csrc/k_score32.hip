@@ -1371,7 +1371,7 @@ const char *score32_variant_name(int v) {
 
 hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     if (a.n_work <= 0) return hipSuccess;
-    if (variant < 0 || variant >= kNumVariants) variant = 0;
+    if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
     // A launch holds fewer than 2^32 threads (2^26 waves); longer work lists
     // are cut into chunks (the kernels address partials by glowworm and
     // block, so a chunk only needs the offset work pointer).
