@@ -327,6 +327,7 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": 1e3 * dt / args.steps,
+        "gso_steps_per_s": args.steps / dt,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
