@@ -5,8 +5,10 @@
 A step is one GSO step (pose transform + DFIRE scoring + luciferin, then
 neighbour selection + movement) over every glowworm of every swarm of the
 workload; inputs and state are resident in HBM before timing starts.  The
-headline form is the batched FP32 guard-band form; the plain FP64 form is
-timed on the same workload for comparison.  Prints one JSON line.
+headline form is the batched FP32 guard-band form, its step captured once as
+a hipGraph (before the timed region) and replayed --steps times; the plain
+FP64 form is timed on the same workload for comparison, and the whole c0 run
+on the CPU oracle and on the GPU.  Prints one JSON line.
 """
 import argparse
 import json
@@ -109,16 +111,32 @@ def ranking_overlap(stream, workload="c1", steps=30):
     return out
 
 
+def host_cpu():
+    """The host's CPU model, its logical CPU count and the CPUs this process
+    may run on."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(cpu_model=model, nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)))
+
+
 def cpu_baseline(wl, threads, target_s=12.0):
     """The CPU oracle (FP64 restatement of the reference CPU step) on a bounded
     sample: one full step (score, select, move) over `threads` swarms of g
-    glowworms of every complex; per-evaluation cost extrapolates to the whole
+    glowworms of every complex, with the workload's own poses and its random
+    stream's first step; per-evaluation cost extrapolates to the whole
     workload."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
     G = wl["G"]
     P = wl["complexes"][0].P
     pos = wl["pos"].reshape(-1, G, P)
+    ustream = wl["stream"].reshape(-1, wl["steps"], G)
     # calibrate g so that the sample takes about target_s
     swarm0 = 0
     cost = 0.0
@@ -133,7 +151,7 @@ def cpu_baseline(wl, threads, target_s=12.0):
     cx, s0, S = samples[0]
     ns = min(threads, S)
     st = O.State(ns, 2, P, O.default_params(), pos[s0:s0 + ns, :2].copy())
-    u = np.full((ns, 1, 2), 0.5)
+    u = ustream[s0:s0 + ns, :1, :2].copy()
     t0 = time.perf_counter()
     O.lib().or_gso_step(cx.c, O.default_params(), st.st, O._p(np.ascontiguousarray(u)), 1, 0, 1,
                         threads)
@@ -147,7 +165,7 @@ def cpu_baseline(wl, threads, target_s=12.0):
     for cx, s0, S in samples:
         ns = min(threads, S)
         st = O.State(ns, g, P, O.default_params(), pos[s0:s0 + ns, :g].copy())
-        u = np.full((ns, 1, g), 0.5)
+        u = ustream[s0:s0 + ns, :1, :g].copy()
         t0 = time.perf_counter()
         O.lib().or_gso_step(cx.c, O.default_params(), st.st, O._p(np.ascontiguousarray(u)), 1, 0,
                             1, threads)
@@ -156,11 +174,57 @@ def cpu_baseline(wl, threads, target_s=12.0):
         evals += ns * g
         step_time += dt / (ns * g) * S * G
     total_evals = sum(S for _, _, S in samples) * G
-    return dict(value=total_evals / step_time, unit="evals/s", cores=threads, kind="port",
-                sample="one GSO step over %d swarms x %d glowworms of each of %d complexes "
-                       "(%d evals, %.1f s); rate extrapolated by per-complex evaluation cost"
-                       % (min(threads, min(S for _, _, S in samples)), g, len(samples), evals,
-                          t_total))
+    out = dict(value=total_evals / step_time, unit="evals/s", cores=threads, kind="port",
+               sample="one GSO step over %d swarms x %d glowworms of each of %d complexes "
+                      "(%d evals, %.1f s; the workload's poses and its stream's first step); "
+                      "rate extrapolated by per-complex evaluation cost"
+                      % (min(threads, min(S for _, _, S in samples)), g, len(samples), evals,
+                         t_total))
+    out.update(host_cpu())
+    return out
+
+
+def cpu_c0_full(threads):
+    """The whole c0 run (757/481 atoms, 130 x 200 glowworms, 7-value poses, 20
+    steps, seed 0) on the CPU oracle with `threads` OpenMP threads, scoring
+    only moved glowworms after the first step as the reference CPU path does
+    (prepare_gso_tasks.cc:434-436); evals/s counts S*G*steps as every form
+    here does."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib as O
+    wl = workloads.build("c0", device=False)
+    inp = wl["inputs"][0]
+    cx = O.Complex(inp["r_xyz"], inp["r_type"], inp["l_xyz"], inp["l_type"], inp["table"],
+                   inp["lig_ref"], inp["r_modes"], inp["l_modes"], inp["r_mask"], inp["l_mask"])
+    S, G, steps = wl["n_swarms"][0], wl["G"], wl["steps"]
+    st = O.State(S, G, cx.P, O.default_params(), wl["pos"])
+    u = np.ascontiguousarray(wl["stream"])
+    scored = 0
+    t0 = time.perf_counter()
+    for k in range(steps):
+        scored += S * G if k == 0 else int(st.get()["moved"].sum())
+        O.lib().or_gso_step(cx.c, O.default_params(), st.st, O._p(u), steps, k, 0, threads)
+    dt = time.perf_counter() - t0
+    out = dict(value=S * G * steps / dt, unit="evals/s", seconds=dt, cores=threads, kind="port",
+               evaluations_performed=scored, evaluations_counted=S * G * steps,
+               workload="c0 in full: 757/481 atoms, 130 x 200 glowworms, 20 steps, FP64")
+    out.update(host_cpu())
+    return out
+
+
+def gpu_c0_full(stream):
+    """The whole c0 run on the GPU in both forms (eager steps, as one run)."""
+    out = {}
+    for name, form in (("fp64", sdgso.FP64), ("fp32_guard", sdgso.FP32_GUARD)):
+        wl = workloads.build("c0")
+        r = sdgso.Run(wl["complexes"], wl["n_swarms"], wl["G"], wl["pos"], wl["stream"],
+                      wl["steps"], form=form)
+        r.set_stream(stream.cuda_stream)
+        dt = timed_steps(r, wl["steps"], graph=False)
+        out[name] = dict(value=wl["n_swarms"][0] * wl["G"] * wl["steps"] / dt, unit="evals/s",
+                         seconds=dt)
+        r.close()
+    return out
 
 
 def traffic_bytes(workload):
@@ -185,6 +249,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-c0", action="store_true",
+                    help="skip the full c0 runs (CPU oracle and both GPU forms)")
     ap.add_argument("--ranking-steps", type=int, default=30,
                     help="steps of the end-to-end FP32-vs-FP64 ranking comparison on c1 (0 = skip)")
     ap.add_argument("--variant", default=None,
@@ -233,8 +299,11 @@ def main():
         return
 
     t0 = time.perf_counter()
+    # stream steps: warm-up, the graph's first (untimed) replay, the timed
+    # replays, the profiled eager steps
+    prof_steps = max(1, min(args.steps, 5))
     wl = workloads.build(args.workload, swarm_scale=args.swarm_scale,
-                         steps=args.warmup + args.steps + 1)
+                         steps=args.warmup + 1 + args.steps + prof_steps)
     n_glow = sum(wl["n_swarms"]) * wl["G"]
     log("workload %s: %d complexes, %d glowworms, built in %.1f s" %
         (args.workload, len(wl["complexes"]), n_glow, time.perf_counter() - t0))
@@ -252,8 +321,29 @@ def main():
     else:
         run.set_variant(args.variant)
         chosen, tune_ms = run.variants(), None
+    # the step graph is captured and instantiated (and replayed once) before
+    # the timed region, which replays it args.steps times; a HIP event pair on
+    # the run's stream brackets every replay
+    run.steps(1, graph=True)
+    run.sync()
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    hip_sync()
+    t1 = time.perf_counter()
+    for e0, e1 in evs:
+        e0.record(stream)
+        run.steps(1, graph=True)
+        e1.record(stream)
+    run.sync()
+    hip_sync()
+    dt = time.perf_counter() - t1
+    replay_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
+    value = n_glow * args.steps / dt
+    # the same step eagerly, with HIP events around every scoring launch (a
+    # graph cannot hold per-launch events that the host reads back without a
+    # synchronisation per replay): the kernel's launch duration
     run.profile(True)
-    dt = timed_steps(run, args.steps, graph=False)
+    dte = timed_steps(run, prof_steps, graph=False)
     prof = run.profile_read()
     guard = run.guard_stats()
     run.profile(False)
@@ -261,11 +351,10 @@ def main():
     filt = run.filter_stats()
     pairs_per_launch = float(st["pairs"].sum())
     score_ms = prof["score_ms"] / max(1, prof["score_launches"])
-    log("fp32 guard: %.2f ms/step, score kernel %.2f ms, pairs/launch %.3e, guard recomputes %d"
-        % (1e3 * dt / args.steps, score_ms, pairs_per_launch, guard["recomputed"]))
-    # graph replay of the same step
-    dtg = timed_steps(run, 1, graph=True) if run.k < wl["steps"] else None
-    value = n_glow * args.steps / dt
+    log("fp32 guard: graph %.2f ms/step (events %.2f), eager %.2f ms/step, score kernel %.2f ms, "
+        "pairs/launch %.3e, guard recomputes %d"
+        % (1e3 * dt / args.steps, float(np.mean(replay_ms)), 1e3 * dte / prof_steps, score_ms,
+           pairs_per_launch, guard["recomputed"]))
 
     # ---- plain FP64 per-complex form on the same workload
     fp64 = None
@@ -297,18 +386,27 @@ def main():
         log("ranking:", rank_cmp)
     log("precision:", prec)
 
-    cpu = None
+    cpu = c0cpu = c0gpu = None
     if not args.no_cpu:
         threads = len(os.sched_getaffinity(0))
         threads = min(threads, int(os.environ.get("OMP_NUM_THREADS", threads)))
         cpu = cpu_baseline(wl, threads)
         log("cpu baseline:", cpu)
+        if not args.no_c0:
+            c0cpu = cpu_c0_full(threads)
+            log("cpu c0 in full:", c0cpu)
+    if not args.no_c0:
+        c0gpu = gpu_c0_full(stream)
+        log("gpu c0 in full:", c0gpu)
 
     flops = 8.0 * pairs_per_launch
     achieved = flops / (score_ms * 1e-3) / 1e12
     # the table reads: 4 bytes per in-cutoff pair against the measured random-gather rate
     probe = sdgso.probe_gather()
     gather_gbs = 4.0 * pairs_per_launch / (score_ms * 1e-3) / 1e9
+    probe_gbs = 4.0 * probe / 1e9
+    gather_frac = gather_gbs / probe_gbs
+    flop_frac = achieved / FP32_PEAK_TFLOPS
     names = sdgso.tune_variants()
     chosen_names = [names[v] for v in chosen]
     # the variant that scores the most work (waves) names the dominant kernel
@@ -328,6 +426,12 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": 1e3 * dt / args.steps,
         "gso_steps_per_s": args.steps / dt,
+        "timing": {"timed": "%d replays of the captured step graph (capture, instantiation and a "
+                            "first replay before t0)" % args.steps,
+                   "replay_ms_events": {"mean": float(np.mean(replay_ms)),
+                                        "min": float(np.min(replay_ms)),
+                                        "max": float(np.max(replay_ms))},
+                   "eager_ms_per_step": 1e3 * dte / prof_steps, "eager_steps": prof_steps},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -337,21 +441,32 @@ def main():
                    "swarms": int(sum(wl["n_swarms"])), "glowworms_per_swarm": wl["G"],
                    "modes": list(wl["spec"]["modes"]), "guard_band_A2": 1e-4,
                    "parallelism": "1 GPU, all swarms of all complexes in one run"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+        # The scoring kernel is bound by its data-dependent 4-byte table
+        # gathers, served by L2 (the 2.26 MB table stays in each XCD's 4 MB
+        # L2); there is no published L2 random-gather figure, so the peak is
+        # the measured rate of a pure gather probe over a table of that size.
+        # frac = max(gather fraction, FP32 flop fraction).
+        "roofline": {"bound": "l2-gather", "achieved": gather_gbs, "peak": probe_gbs,
+                     "unit": "GB/s", "frac": max(gather_frac, flop_frac),
                      "traffic": traffic_bytes(args.workload),
                      "kernel": kernel_name, "variant": variant_name, "launch_ms": score_ms,
-                     "algorithmic": "8 FP32 flop per in-cutoff pair (0 < d2 <= 225), "
+                     "launch_ms_source": "HIP events on the run's stream around each scoring "
+                                         "launch of %d eager steps after the timed replays"
+                                         % prof_steps,
+                     "algorithmic": "4 bytes of table per in-cutoff pair (0 < d2 <= 225), "
                                     "%.4e pairs per launch" % pairs_per_launch,
-                     "gather": {"achieved_GBps": gather_gbs, "probe_GBps": 4.0 * probe / 1e9,
-                                "frac": gather_gbs / (4.0 * probe / 1e9),
-                                "algorithmic": "4 bytes of table per in-cutoff pair, against "
-                                               "the measured random-gather rate over a "
-                                               "2.26 MB table"}},
+                     "peak_source": "sdgso_probe_gather: 4-byte gathers at hashed offsets over "
+                                    "a 2.26 MB table, every CU busy",
+                     "vs_hbm_peak": {"achieved_GBps": gather_gbs, "peak_GBps": 8000.0,
+                                     "frac": gather_gbs / 8000.0},
+                     "flop": {"achieved_TFLOPs": achieved, "peak_TFLOPs": FP32_PEAK_TFLOPS,
+                              "frac": flop_frac,
+                              "algorithmic": "8 FP32 flop per in-cutoff pair (difference-form "
+                                             "d2)"}},
         "cpu_baseline": cpu,
+        "c0_in_full": {"cpu": c0cpu, "gpu": c0gpu},
         "plain_fp64": fp64,
         "speedup_vs_fp64": (value / fp64["value"]) if fp64 else None,
-        "graph_ms_per_step": (1e3 * dtg) if dtg else None,
         "precision": prec,
         "ranking": rank_cmp,
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),

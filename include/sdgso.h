@@ -72,12 +72,18 @@ void sdgso_default_params(sdgso_params *p);
  *   moved by the normal modes, as complex.cc:149-209 sets it),
  *   rec_modes [n_rec_modes][n_rec][3] f64 (NULL when n_rec_modes == 0); the
  *   same for the ligand; lig_ref [3] the ligand reference point;
- *   table [168*168*20] f32, index a*3360 + b*20 + bin. */
+ *   table [168*168*20] f64, index a*3360 + b*20 + bin, as
+ *   FastDifire::difire_energy holds it (fastdfire.h:93, filled at
+ *   prepare_gso_tasks.cc:1175-1189).  The FP64 form gathers these values.  The
+ *   FP32 guard form gathers them rounded to nearest FP32: each term is within
+ *   2^-24 |T| of the FP64 value, so its energy is within 2^-24 * sum |T| (over
+ *   in-cutoff pairs) of the FP64 form's, on top of its summation error; a
+ *   table of FP32 values (the synthetic one) loses nothing. */
 int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type,
                          const int *rec_mask, int n_rec_modes, const double *rec_modes,
                          int n_lig, const double *lig_xyz, const int *lig_type,
                          const int *lig_mask, int n_lig_modes, const double *lig_modes,
-                         const double *lig_ref, const float *table, sdgso_complex *out);
+                         const double *lig_ref, const double *table, sdgso_complex *out);
 int sdgso_complex_destroy(sdgso_complex c);
 
 /* Score n_poses poses [n][7+nr+nl] = (tx,ty,tz, qw,qx,qy,qz, rec extents,

@@ -297,29 +297,30 @@ static int resolve_threads(int threads) {
 void or_phase_score(const or_complex *c, const or_params *p, or_state *st, int rescore_all,
                     int threads) {
     /* prepare_gso_tasks.cc:432-570: score moved (or first-step) glowworms, then
-     * luciferin for every glowworm, then step += 1 */
+     * luciferin for every glowworm, then step += 1.  Each glowworm's work reads
+     * only its own pose and writes only its own fields, so the loop runs over
+     * glowworms (the reference's is over swarms, :432) with the same result;
+     * a run of few swarms then still uses every thread. */
     const int S = st->S, G = st->G, P = st->P;
+    const long n = (long)S * G;
     const int nt = resolve_threads(threads);
 #pragma omp parallel num_threads(nt)
     {
         double *R = (double *)malloc(sizeof(double) * 3 * (size_t)c->nr);
         double *L = (double *)malloc(sizeof(double) * 3 * (size_t)c->nl);
-#pragma omp for schedule(dynamic, 1)
-        for (int s = 0; s < S; s++) {
-            for (int g = 0; g < G; g++) {
-                const size_t id = (size_t)s * G + g;
-                if (rescore_all || st->moved[id] || st->step[id] == 0) {
-                    const double *pose = st->pos + id * P;
-                    or_pose_coords(c, pose, R, L, st->ref + 3 * id);
-                    int64_t np = 0;
-                    const double e = or_dfire_energy(R, c->nr, c->r_type, L, c->nl, c->l_type,
-                                                     c->table, &np);
-                    st->score[id] = or_score_from_energy(e);
-                    if (st->pairs) st->pairs[id] = np;
-                }
-                st->luciferin[id] = (1.0 - p->rho) * st->luciferin[id] + p->gamma * st->score[id];
-                st->step[id] += 1;
+#pragma omp for schedule(dynamic, 4)
+        for (long id = 0; id < n; id++) {
+            if (rescore_all || st->moved[id] || st->step[id] == 0) {
+                const double *pose = st->pos + (size_t)id * P;
+                or_pose_coords(c, pose, R, L, st->ref + 3 * (size_t)id);
+                int64_t np = 0;
+                const double e = or_dfire_energy(R, c->nr, c->r_type, L, c->nl, c->l_type,
+                                                 c->table, &np);
+                st->score[id] = or_score_from_energy(e);
+                if (st->pairs) st->pairs[id] = np;
             }
+            st->luciferin[id] = (1.0 - p->rho) * st->luciferin[id] + p->gamma * st->score[id];
+            st->step[id] += 1;
         }
         free(R);
         free(L);

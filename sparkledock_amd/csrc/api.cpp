@@ -172,7 +172,8 @@ struct sdgso_complex_s {
     std::vector<double> r_xyz, l_xyz;
     DevBuf<double> r0, rmodes, l0, lmodes;
     DevBuf<int> rtype, ltype;
-    DevBuf<float> table;
+    DevBuf<double> table64;    // the caller's table (FP64 form)
+    DevBuf<float> table;       // the same rounded to FP32 (FP32 guard form)
     DevBuf<float4> tiles, lblocks, lhalf;
     DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
@@ -219,8 +220,11 @@ struct sdgso_run_s {
     int64_t n_score = 0, n_move = 0;
 };
 
-static void check_table_and_types(const float *table, const int *rt, int nr, const int *lt, int nl) {
+static void check_table_and_types(const double *table, const int *rt, int nr, const int *lt, int nl) {
     if (!table) fail(SDGSO_EINVAL, "table is null");
+    for (int i = 0; i < SDGSO_TABLE_LEN; i++)
+        if (!std::isfinite(table[i]))
+            fail(SDGSO_EINVAL, "table value at " + std::to_string(i) + " is not finite");
     for (int i = 0; i < nr; i++)
         if (rt[i] < 0 || rt[i] >= SDGSO_NTYPES)
             fail(SDGSO_EINVAL, "receptor type out of [0,168) at atom " + std::to_string(i));
@@ -424,7 +428,7 @@ void sdgso_default_params(sdgso_params *p) {
 int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, const int *rec_mask,
                          int n_rec_modes, const double *rec_modes, int n_lig, const double *lig_xyz,
                          const int *lig_type, const int *lig_mask, int n_lig_modes,
-                         const double *lig_modes, const double *lig_ref, const float *table,
+                         const double *lig_modes, const double *lig_ref, const double *table,
                          sdgso_complex *out) {
     return guarded([&] {
         if (!out) fail(SDGSO_EINVAL, "out is null");
@@ -456,8 +460,12 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         for (int i = 0; i < n_lig; i++) lt[i] = lig_type[i] * SDGSO_NBINS;
         c->rtype.upload(rt);
         c->ltype.upload(lt);
+        // both copies carry the 20-entry zero pad (bin 20 of types (167, 167))
+        std::vector<double> tab64(kTableLen, 0.0);
+        std::memcpy(tab64.data(), table, sizeof(double) * SDGSO_TABLE_LEN);
+        c->table64.upload(tab64);
         std::vector<float> tab(kTableLen, 0.0f);
-        std::memcpy(tab.data(), table, sizeof(float) * SDGSO_TABLE_LEN);
+        for (int i = 0; i < SDGSO_TABLE_LEN; i++) tab[i] = (float)table[i];
         c->table.upload(tab);
         DevComplex &d = c->dev;
         d.nr = n_rec;
@@ -473,6 +481,7 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.rtype = c->rtype.p;
         d.ltype = c->ltype.p;
         d.table = c->table.p;
+        d.table64 = c->table64.p;
         for (int k = 0; k < 3; k++) d.lref[k] = lig_ref[k];
         d.tiles = c->tiles.p;
         d.rmod32 = c->rmod32.p;

@@ -213,9 +213,12 @@ __device__ __forceinline__ void ref_point64(const DevComplex &c, const double *p
 }
 
 // DFIRE contribution of one pair in FP64 (prepare_gso_tasks.cc:1542-1566);
-// returns false when the pair is outside (0, 225].
+// returns false when the pair is outside (0, 225].  The table is the FP64
+// one (FP64 form) or its FP32 copy (the FP32 form's guard-band pairs, whose
+// other terms come from the same copy).
+template <class TableT>
 __device__ __forceinline__ bool pair_term64(double rx, double ry, double rz, double lx, double ly,
-                                            double lz, int ra, int lb, const float *table,
+                                            double lz, int ra, int lb, const TableT *table,
                                             double &val) {
     const double dx = rx - lx;
     const double dy = ry - ly;

@@ -54,7 +54,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     __syncthreads();
 
     const double *ext = pose + 7;
-    const float *table = c.table;
+    const double *table = c.table64;
     double energy = 0.0;  // thread 0 only
     long long npairs = 0;
     const int nchunks = (nr + kScore64Threads - 1) / kScore64Threads;

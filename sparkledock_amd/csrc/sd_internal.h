@@ -8,7 +8,8 @@
 //     lmodes  f64 [nml][3][nl]
 //     rtype   i32 [nr]             type * 3360 (row offset into the table)
 //     ltype   i32 [nl]             type * 20
-//     table   f32 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
+//     table64 f64 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
+//     table   f32 [168*168*20+20]  the same rounded to FP32 (FP32 guard form)
 //   receptor tiles (FP32 guard form): 64 atoms per tile, grouped by bisection
 //     tiles   f32x4 [ntiles]       tile centre (x,y,z) and radius
 //     rrel    f64 [3][ntiles*64]   coordinates - tile centre
@@ -60,7 +61,8 @@ struct DevComplex {
     const double *lmodes;
     const int *rtype;
     const int *ltype;
-    const float *table;
+    const float *table;      // FP32 copy (FP32 guard form)
+    const double *table64;   // the caller's FP64 values (FP64 form)
     double lref[3];
     // receptor tiles
     int ntiles;

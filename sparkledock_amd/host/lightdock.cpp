@@ -16,11 +16,11 @@
 //     with its noxt / noh / now filters, DFIRE-typed, masked, centred, normal
 //     modes from the ANM when use_anm is set, the ligand reference point from
 //     the minimum-volume ellipsoid, starting poses from the reference's
-//     surface sampling with "swarms" given, as FreeSASA is not available to
-//     derive it) or synthetic complexes ("synthetic_receptor_atoms",
+//     surface sampling with "swarms" given, as this build does not link
+//     FreeSASA to derive it) or synthetic complexes ("synthetic_receptor_atoms",
 //     "synthetic_ligand_atoms", "synthetic_seed", seeded placement around
 //     the receptor); the DFIRE table is read from "dfire_table" (one value
-//     per line, the DCparams layout, stored as FP32) or drawn from the
+//     per line, the DCparams layout, kept in FP64) or drawn from the
 //     synthetic N(0,1)*0.5 table;
 //   - every swarm_<i>/gso_<STEPS>.out is written fresh (the reference's mmap
 //     writer appends to stale files, prepare_gso_task.cu:2860-2904);
@@ -325,7 +325,8 @@ int main(int argc, char **argv) {
         check(sdgso_structure_synthetic(nr, rx.data(), &rs), "structure");
         check(sdgso_structure_synthetic(nl, lx.data(), &ls), "structure");
     }
-    std::vector<float> table(SDGSO_TABLE_LEN, 0.0f);
+    // FP64, as FastDifire::difire_energy holds it (fastdfire.h:93)
+    std::vector<double> table(SDGSO_TABLE_LEN, 0.0);
     const std::string table_path = st.text("dfire_table", "");
     if (!table_path.empty() && table_path != "null") {
         std::ifstream tin(resolve(table_path, setup_path));
@@ -335,14 +336,16 @@ int main(int argc, char **argv) {
         }
         double v;
         int n = 0;
-        while (n < SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS && (tin >> v)) table[n++] = (float)v;
+        while (n < SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS && (tin >> v)) table[n++] = v;
         if (n != SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS) {
             std::fprintf(stderr, "the DFIRE table %s holds %d values, expected %d\n",
                          table_path.c_str(), n, SDGSO_NTYPES * SDGSO_NTYPES * SDGSO_NBINS);
             return 1;
         }
     } else {
-        check(sdgso_synth_table(7, table.data()), "synth table");
+        std::vector<float> t32(SDGSO_TABLE_LEN);
+        check(sdgso_synth_table(7, t32.data()), "synth table");
+        for (int i = 0; i < SDGSO_TABLE_LEN; i++) table[i] = t32[i];
     }
     const int P = 7 + nmr + nml;
     if (!from_pdb) {

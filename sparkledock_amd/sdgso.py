@@ -47,7 +47,7 @@ SIGNATURES = [
     ("sdgso_default_params", None, [ctypes.POINTER(Params)]),
     ("sdgso_complex_create", ctypes.c_int,
      [ctypes.c_int, _D, _I, _I, ctypes.c_int, _D, ctypes.c_int, _D, _I, _I, ctypes.c_int, _D, _D,
-      _F, ctypes.POINTER(_VP)]),
+      _D, ctypes.POINTER(_VP)]),
     ("sdgso_complex_destroy", ctypes.c_int, [_VP]),
     ("sdgso_score_poses", ctypes.c_int,
      [_VP, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double, _D, _D, _I64]),
@@ -157,7 +157,8 @@ class Complex:
         rm = None if not self.nmr else np.ascontiguousarray(r_modes, dtype=np.float64)
         lm = None if not self.nml else np.ascontiguousarray(l_modes, dtype=np.float64)
         ref = np.ascontiguousarray(lig_ref, dtype=np.float64).reshape(3)
-        tab = np.ascontiguousarray(table, dtype=np.float32).reshape(-1)
+        # FP64, as the reference's table; an FP32 table converts exactly
+        tab = np.ascontiguousarray(table, dtype=np.float64).reshape(-1)
         if tab.size != TABLE_LEN:
             raise ValueError("table must have 168*168*20 entries")
         self.P = 7 + self.nmr + self.nml
@@ -165,7 +166,7 @@ class Complex:
         _check(lib().sdgso_complex_create(
             self.nr, _p(r_xyz), _p(r_type, _I), _p(r_mask, _I), self.nmr, _p(rm),
             self.nl, _p(l_xyz), _p(l_type, _I), _p(l_mask, _I), self.nml, _p(lm),
-            _p(ref), _p(tab, _F), ctypes.byref(h)))
+            _p(ref), _p(tab), ctypes.byref(h)))
         self.h = h
 
     def close(self):

@@ -77,11 +77,12 @@ def spec(name, swarm_scale=1.0, steps=None):
     return cfg
 
 
-def build(name, swarm_scale=1.0, steps=None, form=None):
+def build(name, swarm_scale=1.0, steps=None, form=None, device=True):
     """Device complexes, poses and streams of a workload.
 
     Returns dict(complexes=[sdgso.Complex], n_swarms, G, pos, stream, steps,
-    form, inputs=[host dicts]).
+    form, inputs=[host dicts]); with device=False the complexes are not
+    created on the device (complexes is empty).
     """
     cfg = spec(name, swarm_scale, steps)
     nmr, nml = cfg["modes"]
@@ -90,9 +91,10 @@ def build(name, swarm_scale=1.0, steps=None, form=None):
     for k, (nr, nl, S) in enumerate(cfg["complexes"]):
         inp = complex_inputs(k, nr, nl, nmr, nml, table)
         inputs.append(inp)
-        cxs.append(sdgso.Complex(inp["r_xyz"], inp["r_type"], inp["l_xyz"], inp["l_type"], table,
-                                 inp["lig_ref"], inp["r_modes"], inp["l_modes"], inp["r_mask"],
-                                 inp["l_mask"]))
+        if device:
+            cxs.append(sdgso.Complex(inp["r_xyz"], inp["r_type"], inp["l_xyz"], inp["l_type"],
+                                     table, inp["lig_ref"], inp["r_modes"], inp["l_modes"],
+                                     inp["r_mask"], inp["l_mask"]))
         ns.append(S)
         pos.append(pose_inputs(k, S, cfg["G"], nr, nl, nmr, nml).reshape(-1))
         st.append(sdgso.stream(0, S, cfg["steps"], cfg["G"]).reshape(-1))

@@ -141,6 +141,37 @@ def test_energy_with_terms_of_both_signs_over_eight_decades():
         assert_energy((4.7 - s) / 0.0157, e0, denom, v)
 
 
+def test_fp64_table_values():
+    """The table enters in FP64, as the reference's FastDifire holds it
+    (fastdfire.h:93): with values that FP32 cannot represent (decimals of four
+    places, as a DCparams text file gives them), the FP64 form is bit-exact to
+    the oracle.  The FP32 form gathers the values rounded to FP32: its energy
+    is within its summation tolerance of the oracle on the rounded table, and
+    within 2^-24 sum |T| more of the oracle on the FP64 table."""
+    c = GC.case(33, 700, 300, 10, 10, 3, 16)
+    rng = np.random.default_rng(33)
+    c["table"] = np.round(rng.normal(size=168 * 168 * 20) * 0.5, 4)
+    t32 = c["table"].astype(np.float32)
+    assert np.mean(t32.astype(np.float64) != c["table"]) > 0.99
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, ref0, pairs0 = pr.oracle_scores(poses)
+    s32tab, _, _ = pr.oracle_scores(poses, table=t32)
+    assert pairs0.min() > 0 and not np.array_equal(s0, s32tab)
+    s, ref, pairs = pr.gpu.score(poses, sdgso.FP64)
+    assert np.array_equal(pairs, pairs0)
+    assert np.array_equal(ref, ref0)
+    assert np.array_equal(s, s0)
+    denom = GC.sum_abs_terms(pr, poses)
+    e0, e0_32 = (4.7 - s0) / 0.0157, (4.7 - s32tab) / 0.0157
+    for v in VARIANTS:
+        s, pairs = score32(pr.gpu, poses, v)
+        assert np.array_equal(pairs, pairs0), v
+        e = (4.7 - s) / 0.0157
+        assert_energy(e, e0_32, denom, v)
+        assert np.all(np.abs(e - e0) <= REL_E * np.abs(e0) + (FLOOR_E + 2.0 ** -24) * denom), v
+
+
 def test_bins_exact_on_lattice_boundaries():
     """Lattice geometry puts many pairs exactly on k^2/4 boundaries, on the
     225 A^2 cutoff and at d2 = 0: both forms must bin them as the oracle."""
