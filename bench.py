@@ -270,7 +270,10 @@ def main():
     if not torch.cuda.is_available():
         sys.exit("no GPU visible")
     torch.cuda.set_device(0)
-    stream = torch.cuda.current_stream()
+    # a stream of our own (default priority) as torch's current stream: the
+    # runs launch on it and torch's synchronisation covers it
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
     # the library is loaded after torch has set up the device
     if args.variant is not None:
         names = sdgso.tune_variants()
@@ -299,11 +302,11 @@ def main():
         return
 
     t0 = time.perf_counter()
-    # stream steps: warm-up, the graph's first (untimed) replay, the timed
-    # replays, the profiled eager steps
-    prof_steps = max(1, min(args.steps, 5))
+    # stream steps: warm-up, the graphs' first (untimed) replays, the timed
+    # replays, then replays of the one-graph step and eager steps to compare
+    cmp_steps = max(1, min(args.steps, 5))
     wl = workloads.build(args.workload, swarm_scale=args.swarm_scale,
-                         steps=args.warmup + 1 + args.steps + prof_steps)
+                         steps=args.warmup + 2 + args.steps + 2 * cmp_steps)
     n_glow = sum(wl["n_swarms"]) * wl["G"]
     log("workload %s: %d complexes, %d glowworms, built in %.1f s" %
         (args.workload, len(wl["complexes"]), n_glow, time.perf_counter() - t0))
@@ -321,40 +324,33 @@ def main():
     else:
         run.set_variant(args.variant)
         chosen, tune_ms = run.variants(), None
-    # the step graph is captured and instantiated (and replayed once) before
-    # the timed region, which replays it args.steps times; a HIP event pair on
-    # the run's stream brackets every replay
+    # Timed region: args.steps replays of the captured step, with profiling
+    # on, so that each replay is the scoring graph and the graph of the rest
+    # of the step with HIP events on the run's stream between them (the
+    # kernel's launch durations come from the timed region itself).  Both
+    # graphs are captured, instantiated and replayed once before t0.
+    run.profile(True)
     run.steps(1, graph=True)
     run.sync()
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(args.steps)]
-    hip_sync()
-    t1 = time.perf_counter()
-    for e0, e1 in evs:
-        e0.record(stream)
-        run.steps(1, graph=True)
-        e1.record(stream)
-    run.sync()
-    hip_sync()
-    dt = time.perf_counter() - t1
-    replay_ms = [e0.elapsed_time(e1) for e0, e1 in evs]
-    value = n_glow * args.steps / dt
-    # the same step eagerly, with HIP events around every scoring launch (a
-    # graph cannot hold per-launch events that the host reads back without a
-    # synchronisation per replay): the kernel's launch duration
-    run.profile(True)
-    dte = timed_steps(run, prof_steps, graph=False)
+    run.profile(True)  # drop the untimed replay's events
+    dt = timed_steps(run, args.steps, graph=True)
     prof = run.profile_read()
     guard = run.guard_stats()
     run.profile(False)
+    value = n_glow * args.steps / dt
     st = run.read()
-    filt = run.filter_stats()
     pairs_per_launch = float(st["pairs"].sum())
     score_ms = prof["score_ms"] / max(1, prof["score_launches"])
-    log("fp32 guard: graph %.2f ms/step (events %.2f), eager %.2f ms/step, score kernel %.2f ms, "
-        "pairs/launch %.3e, guard recomputes %d"
-        % (1e3 * dt / args.steps, float(np.mean(replay_ms)), 1e3 * dte / prof_steps, score_ms,
-           pairs_per_launch, guard["recomputed"]))
+    rest_ms = prof["move_ms"] / max(1, prof["move_launches"])
+    # the same step as one graph (no events) and eagerly, for comparison
+    run.steps(1, graph=True)
+    dt1 = timed_steps(run, cmp_steps, graph=True)
+    dte = timed_steps(run, cmp_steps, graph=False)
+    filt = run.filter_stats()
+    log("fp32 guard: %.2f ms/step (profiled graph replays: scoring %.2f ms + rest %.2f ms); "
+        "one-graph replays %.2f ms/step, eager %.2f ms/step; pairs/launch %.3e, guard "
+        "recomputes %d" % (1e3 * dt / args.steps, score_ms, rest_ms, 1e3 * dt1 / cmp_steps,
+                           1e3 * dte / cmp_steps, pairs_per_launch, guard["recomputed"]))
 
     # ---- plain FP64 per-complex form on the same workload
     fp64 = None
@@ -426,12 +422,13 @@ def main():
         "warmup": args.warmup,
         "ms_per_step": 1e3 * dt / args.steps,
         "gso_steps_per_s": args.steps / dt,
-        "timing": {"timed": "%d replays of the captured step graph (capture, instantiation and a "
-                            "first replay before t0)" % args.steps,
-                   "replay_ms_events": {"mean": float(np.mean(replay_ms)),
-                                        "min": float(np.min(replay_ms)),
-                                        "max": float(np.max(replay_ms))},
-                   "eager_ms_per_step": 1e3 * dte / prof_steps, "eager_steps": prof_steps},
+        "timing": {"timed": "%d replays of the captured step (capture, instantiation and a "
+                            "first replay before t0), each as the scoring graph and the graph "
+                            "of the rest of the step with HIP events between them"
+                            % args.steps,
+                   "scoring_ms": score_ms, "rest_ms": rest_ms,
+                   "one_graph_ms_per_step": 1e3 * dt1 / cmp_steps,
+                   "eager_ms_per_step": 1e3 * dte / cmp_steps, "compare_steps": cmp_steps},
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -450,9 +447,8 @@ def main():
                      "unit": "GB/s", "frac": max(gather_frac, flop_frac),
                      "traffic": traffic_bytes(args.workload),
                      "kernel": kernel_name, "variant": variant_name, "launch_ms": score_ms,
-                     "launch_ms_source": "HIP events on the run's stream around each scoring "
-                                         "launch of %d eager steps after the timed replays"
-                                         % prof_steps,
+                     "launch_ms_source": "HIP events on the run's stream around the scoring "
+                                         "graph of each timed replay",
                      "algorithmic": "4 bytes of table per in-cutoff pair (0 < d2 <= 225), "
                                     "%.4e pairs per launch" % pairs_per_launch,
                      "peak_source": "sdgso_probe_gather: 4-byte gathers at hashed offsets over "

@@ -110,7 +110,9 @@ int sdgso_run_destroy(sdgso_run r);
 int sdgso_run_set_stream(sdgso_run r, void *hip_stream);
 
 /* Advance n steps (asynchronous on the run's stream).  use_graph != 0 replays
- * the step as a captured hipGraph. */
+ * the step as a captured hipGraph (captured at the first such call; while
+ * profiling is enabled, as two graphs, the scoring launches and the rest of
+ * the step, with the timing events between them). */
 int sdgso_run_steps(sdgso_run r, int n, int use_graph);
 
 /* Phases of one step, for single-phase tests: bit 0 score + luciferin,
@@ -133,9 +135,11 @@ int sdgso_run_read(sdgso_run r, double *pos, double *luciferin, double *score, d
 int sdgso_run_write(sdgso_run r, const double *pos, const double *luciferin, const double *score,
                     const double *vision, const int *moved, const double *ref);
 
-/* Per-phase HIP-event timing on the run's stream.  enable != 0 starts
- * recording; read returns totals in milliseconds and launch counts since the
- * last enable: times[0] scoring, times[1] select + move. */
+/* Per-phase HIP-event timing on the run's stream, for eager steps and graph
+ * replays alike.  enable != 0 starts recording; read returns totals in
+ * milliseconds and step counts since the last enable: times[0] the scoring
+ * launches, times[1] the rest of the step (the FP32 form's per-glowworm
+ * epilogue, selection + movement). */
 int sdgso_run_profile(sdgso_run r, int enable);
 int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches);
 

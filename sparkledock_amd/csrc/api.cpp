@@ -208,7 +208,9 @@ struct sdgso_run_s {
     DevBuf<unsigned long long> guard_count;
     hipStream_t own_stream = nullptr;
     hipStream_t stream_h = nullptr;
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph = nullptr;        // one step
+    hipGraphExec_t graph_score = nullptr;  // profiled replays: the scoring launches,
+    hipGraphExec_t graph_rest = nullptr;   //   then the rest of the step
     bool profiling = false;
     // FP32 form: scoring-kernel variant of each complex (sdgso_run_tune) and
     // the complex's range [work0[k], work0[k+1]) of the work list (work2_0:
@@ -570,17 +572,14 @@ static void launch_score32_run(sdgso_run r, const ScoreArgs &a) {
     }
 }
 
-// The scoring kernel (score64 or score32) sits between the two optional
-// events; finish32 (a per-glowworm epilogue) runs after them.
-static void enqueue_score(sdgso_run r, hipEvent_t ev0 = nullptr, hipEvent_t ev1 = nullptr) {
-    ScoreArgs a = score_args(r);
-    if (ev0) HIPCHK(hipEventRecord(ev0, r->stream_h));
+// The scoring phase: the scoring kernel (score64, or score32 and its
+// per-glowworm epilogue finish32), luciferin included.
+static void enqueue_score(sdgso_run r) {
+    const ScoreArgs a = score_args(r);
     if (r->form == SDGSO_FP64) {
         HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
-        if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
     } else {
         launch_score32_run(r, a);
-        if (ev1) HIPCHK(hipEventRecord(ev1, r->stream_h));
         HIPCHK(launch_finish32(a, r->stream_h));
     }
 }
@@ -614,22 +613,70 @@ static void enqueue_select_move(sdgso_run r, int do_select, int do_move) {
     HIPCHK(launch_select_move(m, r->stream_h));
 }
 
-static void enqueue_step(sdgso_run r, bool timed) {
-    std::pair<hipEvent_t, hipEvent_t> e1{}, e2{};
-    if (timed) {
-        profile_pair(r, r->ev_score, e1);
-        profile_pair(r, r->ev_move, e2);
-    }
-    enqueue_score(r, e1.first, e1.second);
-    if (timed) HIPCHK(hipEventRecord(e2.first, r->stream_h));
+// A step in two parts: the scoring launches, and the rest (the FP32 form's
+// per-glowworm epilogue, selection + movement, the device step counter).
+static void enqueue_score_launches(sdgso_run r, const ScoreArgs &a) {
+    if (r->form == SDGSO_FP64)
+        HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
+    else
+        launch_score32_run(r, a);
+}
+
+static void enqueue_rest(sdgso_run r, const ScoreArgs &a) {
+    if (r->form != SDGSO_FP64) HIPCHK(launch_finish32(a, r->stream_h));
     enqueue_select_move(r, 1, 1);
-    if (timed) HIPCHK(hipEventRecord(e2.second, r->stream_h));
     HIPCHK(launch_advance(r->d_k.p, r->stream_h));
 }
 
+// Events around the two parts of the next step (profiling).
+static void step_events(sdgso_run r, std::pair<hipEvent_t, hipEvent_t> &e1,
+                        std::pair<hipEvent_t, hipEvent_t> &e2) {
+    profile_pair(r, r->ev_score, e1);
+    profile_pair(r, r->ev_move, e2);
+}
+
+static void enqueue_step(sdgso_run r, bool timed) {
+    std::pair<hipEvent_t, hipEvent_t> e1{}, e2{};
+    if (timed) step_events(r, e1, e2);
+    const ScoreArgs a = score_args(r);
+    if (timed) HIPCHK(hipEventRecord(e1.first, r->stream_h));
+    enqueue_score_launches(r, a);
+    if (timed) {
+        HIPCHK(hipEventRecord(e1.second, r->stream_h));
+        HIPCHK(hipEventRecord(e2.first, r->stream_h));
+    }
+    enqueue_rest(r, a);
+    if (timed) HIPCHK(hipEventRecord(e2.second, r->stream_h));
+}
+
+// Captures part of a step (0: all of it, 1: the scoring launches, 2: the
+// rest) as an executable graph.  The stream index lives on the device
+// (advanced by the last node), so every replay is the same graph.
+static hipGraphExec_t capture_step(sdgso_run r, int part) {
+    hipGraph_t g = nullptr;
+    HIPCHK(hipStreamBeginCapture(r->stream_h, hipStreamCaptureModeThreadLocal));
+    try {
+        const ScoreArgs a = score_args(r);
+        if (part != 2) enqueue_score_launches(r, a);
+        if (part != 1) enqueue_rest(r, a);
+    } catch (...) {
+        // leave the stream out of capture mode before reporting
+        if (hipStreamEndCapture(r->stream_h, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHK(hipStreamEndCapture(r->stream_h, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) fail(SDGSO_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    return ex;
+}
+
 static void drop_graph(sdgso_run r) {
-    if (r->graph) (void)hipGraphExecDestroy(r->graph);
-    r->graph = nullptr;
+    for (hipGraphExec_t *g : {&r->graph, &r->graph_score, &r->graph_rest}) {
+        if (*g) (void)hipGraphExecDestroy(*g);
+        *g = nullptr;
+    }
 }
 
 static void harvest_profile(sdgso_run r) {
@@ -818,23 +865,29 @@ int sdgso_run_steps(sdgso_run r, int n, int use_graph) {
             fail(SDGSO_ESTATE, "stepping past the random stream (" + std::to_string(r->n_steps) +
                                    " steps provided)");
         ensure_device();
-        if (use_graph && !r->profiling && n > 0) {
-            // One step captured once; the stream index lives on the device
-            // (advanced by the last node), so every replay is the same graph.
-            if (!r->graph) {
-                hipGraph_t g = nullptr;
-                HIPCHK(hipStreamBeginCapture(r->stream_h, hipStreamCaptureModeThreadLocal));
-                enqueue_step(r, false);
-                HIPCHK(hipStreamEndCapture(r->stream_h, &g));
-                const hipError_t e = hipGraphInstantiate(&r->graph, g, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(g);
-                if (e != hipSuccess) {
-                    r->graph = nullptr;
-                    fail(SDGSO_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        if (use_graph && n > 0) {
+            if (!r->profiling) {
+                // one step captured once, replayed n times
+                if (!r->graph) r->graph = capture_step(r, 0);
+                for (int i = 0; i < n; i++) {
+                    HIPCHK(hipGraphLaunch(r->graph, r->stream_h));
+                    r->k++;
                 }
+                return;
             }
+            // profiled replays: the step as two graphs with the timing events
+            // between them (fresh events per step, read back later)
+            if (!r->graph_score) r->graph_score = capture_step(r, 1);
+            if (!r->graph_rest) r->graph_rest = capture_step(r, 2);
             for (int i = 0; i < n; i++) {
-                HIPCHK(hipGraphLaunch(r->graph, r->stream_h));
+                std::pair<hipEvent_t, hipEvent_t> e1{}, e2{};
+                step_events(r, e1, e2);
+                HIPCHK(hipEventRecord(e1.first, r->stream_h));
+                HIPCHK(hipGraphLaunch(r->graph_score, r->stream_h));
+                HIPCHK(hipEventRecord(e1.second, r->stream_h));
+                HIPCHK(hipEventRecord(e2.first, r->stream_h));
+                HIPCHK(hipGraphLaunch(r->graph_rest, r->stream_h));
+                HIPCHK(hipEventRecord(e2.second, r->stream_h));
                 r->k++;
             }
             return;

@@ -442,16 +442,23 @@ def test_every_graph_replay_equals_eager():
         eager, _ = make_run(pr, c, 5, form=form)
         replays, _ = make_run(pr, c, 5, form=form)
         mixed, _ = make_run(pr, c, 5, form=form)
+        # profiled replays: the step as two graphs with timing events between
+        profiled, _ = make_run(pr, c, 5, form=form)
+        profiled.profile(True)
         for k in range(5):
             eager.steps(1)
             replays.steps(1, graph=True)
             mixed.steps(1, graph=(k % 2 == 0))
+            profiled.steps(1, graph=True)
             want = eager.read()
-            for run in (replays, mixed):
+            for run in (replays, mixed, profiled):
                 got = run.read()
                 for key in keys:
                     assert np.array_equal(got[key], want[key]), (form, k, key)
-        for run in (eager, replays, mixed):
+        prof = profiled.profile_read()
+        assert prof["score_launches"] == prof["move_launches"] == 5
+        assert prof["score_ms"] > 0 and prof["move_ms"] > 0
+        for run in (eager, replays, mixed, profiled):
             run.close()
 
 
