@@ -94,6 +94,26 @@ int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int for
                       double guard_band, double *out_score, double *out_ref,
                       int64_t *out_pairs);
 
+/* The scoring call without allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It scores
+ * as sdgso_score_poses does, with the same kernels, but poses [n][7+nr+nl],
+ * out_score [n], out_ref [n][3] and out_pairs [n] are DEVICE pointers
+ * (out_ref and out_pairs may be NULL), the work is enqueued on hip_stream
+ * (a hipStream_t; NULL = the default stream) without waiting for it, and the
+ * scratch the kernels need comes from one static device array of
+ * SDGSO_NOALLOC_SCRATCH_MB MiB per GPU, held until the process ends.  A call
+ * whose scratch would exceed it (per pose 4 bytes, 24 more without out_ref,
+ * and in the FP32 form 16 * nb + 8 * ceil(nb / 2) more, nb = ceil(nl / 64)
+ * ligand blocks; plus 256-byte alignment of each region) fails with
+ * SDGSO_EINVAL and a
+ * message giving the most poses that fit.  Every call on a GPU shares that
+ * GPU's array: calls that overlap on one GPU race. */
+#define SDGSO_NOALLOC_SCRATCH_MB 64
+int sdgso_score_poses_noalloc(sdgso_complex c, int n_poses, const double *poses, int form,
+                              double guard_band, double *out_score, double *out_ref,
+                              int64_t *out_pairs, void *hip_stream);
+
 /* A GSO run over one or more complexes sharing the glowworm count and pose
  * length.  Complex k owns swarms [sum_{j<k} n_swarms[j], ...).
  *   pos    [sum S][G][P] initial poses

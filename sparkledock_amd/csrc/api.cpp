@@ -185,6 +185,7 @@ struct sdgso_complex_s {
     DevBuf<double> lblk;       // [nlblocks][3][64] ligand coordinates by block slot
     DevBuf<float> lblkm;       // [nlblocks][3*nml][64] ligand modes by block slot, FP32
     DevComplex dev{};          // descriptor with device pointers
+    DevBuf<DevComplex> dself;  // the same descriptor in device memory
     double max_rec_extent = 0; // |xyz| maxima, for the guard-band bound
 };
 
@@ -407,6 +408,24 @@ static void build_lblocks(sdgso_complex c, const double *xyz, const std::vector<
     c->dev.nlblocks = nb;
 }
 
+namespace sd {
+
+int use_device() { return guarded([] { ensure_device(); }); }
+
+bool complex_view(sdgso_complex c, ComplexView *v) {
+    if (!c || !v) return false;
+    v->dev = c->dself.p;
+    v->nr = c->nr;
+    v->nl = c->nl;
+    v->nmr = c->nmr;
+    v->nml = c->nml;
+    v->nlblocks = c->dev.nlblocks;
+    v->ntiles = c->dev.ntiles;
+    return true;
+}
+
+}  // namespace sd
+
 extern "C" {
 
 const char *sdgso_last_error(void) { return g_error.c_str(); }
@@ -499,6 +518,7 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.lblk = c->lblk.p;
         d.lblkm = c->lblkm.p;
         d.rrel = c->rrel.p;
+        c->dself.upload(&d, 1);
         *out = c.release();
     });
 }

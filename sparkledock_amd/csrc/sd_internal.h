@@ -170,4 +170,16 @@ hipError_t launch_gather_probe(const float *table, unsigned n, int per_thread, f
 
 void set_error(const std::string &msg);
 
+// Checks for a usable gfx950 device and makes it current (SDGSO_OK or an
+// error status with the message set).
+int use_device();
+
+// What other translation units need of a complex: its device-resident
+// descriptor and sizes.
+struct ComplexView {
+    const DevComplex *dev;
+    int nr, nl, nmr, nml, nlblocks, ntiles;
+};
+bool complex_view(sdgso_complex c, ComplexView *v);
+
 }  // namespace sd

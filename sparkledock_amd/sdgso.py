@@ -51,6 +51,9 @@ SIGNATURES = [
     ("sdgso_complex_destroy", ctypes.c_int, [_VP]),
     ("sdgso_score_poses", ctypes.c_int,
      [_VP, ctypes.c_int, _D, ctypes.c_int, ctypes.c_double, _D, _D, _I64]),
+    # device pointers and a hipStream_t
+    ("sdgso_score_poses_noalloc", ctypes.c_int,
+     [_VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_double, _VP, _VP, _VP, _VP]),
     ("sdgso_run_create", ctypes.c_int,
      [ctypes.c_int, ctypes.POINTER(_VP), _I, ctypes.c_int, _D, _D, ctypes.c_int,
       ctypes.POINTER(Params), ctypes.c_int, ctypes.c_double, ctypes.POINTER(_VP)]),
