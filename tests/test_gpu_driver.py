@@ -1,8 +1,9 @@
 """The lightdock driver end to end (`lightdock -f setup.json -s STEPS`, the
 reference CLI of source/simulation/run_simulation.cc): it writes one
 swarm_<i>/gso_<STEPS>.out per swarm in the reference line format
-(prepare_gso_task.cu:2868-2888) and the values equal the same run through the
-C ABI; it clusters every swarm into swarm_<i>/cluster.repr
+(prepare_gso_task.cu:2868-2888), the values equal the same run through the
+C ABI, and in the FP64 form its files equal the oracle's run byte for byte;
+it clusters every swarm into swarm_<i>/cluster.repr
 (prepare_gso_tasks.cc:1084-1110) and optionally writes the final poses as
 PDB files; it reads PDB inputs (the 2X9A benchmark complex, tests/golden/pdb)."""
 import json
@@ -84,6 +85,49 @@ def test_lightdock_writes_swarm_files(tmp_path, form):
         for c, size, score, r in rows:
             assert size == int(np.sum(cl[s] == c)) and rep[s][cl[s] == c][0] == r
             assert abs(score - got["score"][s * G + r]) <= 1e-5 * max(1.0, abs(score))
+
+
+def reference_line(pose, lucif, nnei, vis, score):
+    """One glowworm's line as the reference writer builds it
+    (prepare_gso_task.cu:2868-2888): std::to_string of every value, which
+    prints a double as "%f"."""
+    return ("(" + ", ".join("%f" % v for v in pose) + ")\t0\t0\t" + "%f\t%d\t%f\t%f\t" %
+            (lucif, nnei, vis, score))
+
+
+def test_lightdock_files_equal_oracle_run(tmp_path):
+    """The driver's files against the oracle: lightdock in the FP64 form, and
+    the oracle's run of the same inputs (the CPU policy, re-scoring moved
+    glowworms only), formatted by the reference writer's rules; every line of
+    every swarm_<i>/gso_<S>.out equal byte for byte."""
+    import oracle_lib as O
+    nr, nl, S, G, nm, steps, seed = 400, 150, 3, 24, 4, 6, 5
+    setup = {"swarms": S, "glowworms": G, "use_anm": True, "anm_rec": nm, "anm_lig": nm,
+             "swarm_radius": 3.0, "synthetic_receptor_atoms": nr, "synthetic_ligand_atoms": nl,
+             "synthetic_seed": seed, "form": "fp64"}
+    (tmp_path / "setup.json").write_text(json.dumps(setup))
+    r = subprocess.run([DRIVER, "-f", "setup.json", "-s", str(steps)], cwd=tmp_path,
+                       stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    # the inputs the driver derives from the set-up (host functions only)
+    rx, rt, rm = sdgso.synth_molecule(1000 * seed + 1, nr, nm)
+    lx, lt, lm = sdgso.synth_molecule(1000 * seed + 2, nl, nm)
+    centre = math.pow(nr * 11.4, 1 / 3) + 0.5 * math.pow(nl * 11.4, 1 / 3)
+    pos = sdgso.synth_poses(1000 * seed + 3, S, G, nm, nm, centre, 3.0)
+    ref = [0.0, 0.0, 0.0]
+    for i in range(nl):
+        for k in range(3):
+            ref[k] += float(lx[i, k]) / nl
+    ora = O.Complex(rx, rt, lx, lt, sdgso.synth_table(7), ref, rm, lm)
+    st = O.run(ora, O.default_params(), pos, O.stream(0, S, steps, G), steps, S, G,
+               rescore_all=False).get()
+    assert st["moved"].sum() > 0
+    for s in range(S):
+        got = (tmp_path / ("swarm_%d" % s) / ("gso_%d.out" % steps)).read_bytes().decode()
+        want = ranking.HEADER + "\n" + "".join(
+            reference_line(st["pos"][s, g], st["luciferin"][s, g], st["nnei"][s, g],
+                           st["vision"][s, g], st["score"][s, g]) + "\n" for g in range(G))
+        assert got == want, s
 
 
 def test_lightdock_usage_errors(tmp_path):

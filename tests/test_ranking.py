@@ -24,3 +24,20 @@ def test_rank_and_overlap():
     b[-10:] = -1.0  # the 10 best of a are the 10 worst of b
     assert ranking.top_overlap(a, a, 100) == 1.0
     assert ranking.top_overlap(a, b, 100) == 0.9
+
+
+def test_percent_f_equals_c_printf():
+    """The driver test formats the oracle's values with Python's "%f" as the
+    reference's std::to_string(double) does (C++11 [string.conversions]:
+    sprintf "%f"): the two agree on random values of every magnitude, on
+    halfway cases of the sixth decimal, on signed zeros and tiny negatives."""
+    import ctypes
+    libc = ctypes.CDLL(None)
+    buf = ctypes.create_string_buffer(512)
+    rng = np.random.default_rng(3)
+    vals = list(rng.normal(size=2000) * 10.0 ** rng.uniform(-9, 9, 2000))
+    vals += [0.0, -0.0, -1e-7, 1e-7, 0.5e-6, -0.5e-6, 1.5e-6, 2.5e-6, 0.0000005, 123456.0000005,
+             1e300, -1e300, 5e-324]
+    for v in vals:
+        libc.snprintf(buf, 512, b"%f", ctypes.c_double(v))
+        assert buf.value.decode() == "%f" % v, v
