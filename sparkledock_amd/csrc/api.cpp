@@ -188,6 +188,8 @@ struct sdgso_complex_s {
     DevBuf<int> rtype, ltype;
     DevBuf<double> table64;    // the caller's table (FP64 form)
     DevBuf<float> table;       // the same rounded to FP32 (FP32 guard form)
+    DevBuf<float> tablep;      // the FP32 values as [a][bin][b] (FP32 form's gathers)
+    DevBuf<int> ltype_b;       // ligand types
     DevBuf<float4> tiles, lblocks, lhalf;
     DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
@@ -308,7 +310,7 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
             if (s < a1) {
                 const int i = order[s];
                 perm[s] = i;
-                ts[s] = type[i] * SDGSO_NBINS * SDGSO_NTYPES;
+                ts[s] = type[i] * kPRow;
                 for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = xyz[3 * (size_t)i + k] - (double)cf[k];
                 for (int m = 0; m < nm; m++)
                     for (int k = 0; k < 3; k++) {
@@ -505,6 +507,14 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         std::vector<float> tab(kTableLen, 0.0f);
         for (int i = 0; i < SDGSO_TABLE_LEN; i++) tab[i] = (float)table[i];
         c->table.upload(tab);
+        std::vector<float> tabp(kTableLenP, 0.0f);
+        for (int ta = 0; ta < SDGSO_NTYPES; ta++)
+            for (int bin = 0; bin <= SDGSO_NBINS; bin++)
+                for (int tb = 0; tb < SDGSO_NTYPES; tb++)
+                    tabp[(size_t)ta * kPRow + bin * SDGSO_NTYPES + tb] =
+                        tab[(size_t)ta * SDGSO_NTYPES * SDGSO_NBINS + tb * SDGSO_NBINS + bin];
+        c->tablep.upload(tabp);
+        c->ltype_b.upload(lig_type, n_lig);
         DevComplex &d = c->dev;
         d.nr = n_rec;
         d.nl = n_lig;
@@ -520,6 +530,8 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.ltype = c->ltype.p;
         d.table = c->table.p;
         d.table64 = c->table64.p;
+        d.tablep = c->tablep.p;
+        d.ltype_b = c->ltype_b.p;
         for (int k = 0; k < 3; k++) d.lref[k] = lig_ref[k];
         d.tiles = c->tiles.p;
         d.rmod32 = c->rmod32.p;

@@ -68,7 +68,8 @@ constexpr float kQ2 = 1099511627776.0f;       // 2^40
 constexpr float k225q = 225.0f * kQ2;         // the cutoff in 2^-40 A^2 units (exact)
 constexpr float kFilterMargin = 0.05f;        // A^2, over the MFMA filter's rounding
 constexpr int kFar = 1 << 29;                 // far-away fixed-point coordinate (512 A)
-constexpr int kZeroIdx = SDGSO_TABLE_LEN + SDGSO_NBINS - 1;  // a zero of the table pad
+// The table gathers read tablep ([a][bin][b]): row a*kPRow from the receptor
+// atom, b from the ligand atom, then bin*168.
 
 // Compensated FP32 sum (Neumaier's form of Kahan's method, with Knuth's
 // branch-free TwoSum for each term's rounding error): sum + comp is the exact
@@ -219,7 +220,7 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
         double px, py, pz;
         ligand_slot_grid(c, x.pose, x.rot, b, x.lane, px, py, pz);
         x.L = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
-                        gld(c.ltype, x.lig));
+                        gld(c.ltype_b, x.lig));
     }
     x.nblk = __popcll(__ballot(x.lig >= 0));
     if (x.lane < c.nmr) ef[x.lane] = (float)rext[x.lane];
@@ -337,7 +338,7 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
 template <int kUnroll, int kMinWaves>
 __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(ScoreArgs a) {
     __shared__ float s_d2[kWaves][kQueueR];   // queued pairs: fixed-point d2
-    __shared__ int s_row[kWaves][kQueueR];    //   table row (a*3360 + b*20)
+    __shared__ int s_row[kWaves][kQueueR];    //   tablep offset a*kPRow + b
     __shared__ int s_tag[kWaves][kQueueR];    //   (ligand lane << 24) | receptor atom
     __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
     __shared__ unsigned short s_ct[kWaves][kMaxTiles];
@@ -382,7 +383,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(Sc
                 const PairClass pc = classify(d2q, bandq);
                 near[u] = valid && pc.near;
                 take[u] = valid && pc.take;
-                v[u] = gld(c.table, take[u] ? row + pc.bin : kZeroIdx);
+                v[u] = gld(c.tablep, take[u] ? row + SDGSO_NTYPES * pc.bin : kZeroIdxP);
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
@@ -522,7 +523,7 @@ __global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(Sc
                 const PairClass pc = classify(d2_fixed(l, r), bandq);
                 near[u] = valid && pc.near;
                 take[u] = valid && pc.take;
-                v[u] = gld(c.table, take[u] ? r.w + l.w + pc.bin : kZeroIdx);
+                v[u] = gld(c.tablep, take[u] ? r.w + l.w + SDGSO_NTYPES * pc.bin : kZeroIdxP);
             }
 #pragma unroll
             for (int u = 0; u < kUnroll; u++) {
@@ -760,7 +761,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     const double *rrel4 = uniform_ptr(c.rrel4);
     const float4 *rmodA = uniform_ptr((const float4 *)c.rmodA);
     const __amdgpu_buffer_rsrc_t tbl = __builtin_amdgcn_make_buffer_rsrc(
-        (void *)uniform_ptr(c.table), (short)0, kTableLen * 4, 0x00020000);
+        (void *)uniform_ptr(c.tablep), (short)0, kTableLenP * 4, 0x00020000);
     int4 *rt = s_rt[w];
     float4 *rf = s_rf[w];
     unsigned short *rs = s_rs[w];
@@ -795,7 +796,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 double px, py, pz;
                 ligand_slot_grid(c, x.pose, x.rot, b2, lane, px, py, pz);
                 LB = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
-                               gld(c.ltype, lig_b));
+                               gld(c.ltype_b, lig_b));
             }
             fb[0] = uniform_f32((float)cx);
             fb[1] = uniform_f32((float)cy);
@@ -906,7 +907,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         const int4 r = *(const int4 *)(rt_base + ((unsigned)tag >> 16));
         const int4 l = *(const int4 *)(lg_base + (tag & 0xffff));
         const PairClass pc = classify(d2_fixed(l, r), bandq);
-        const unsigned off = pc.take ? 4u * (unsigned)(r.w + l.w + pc.bin) : kNoLoad;
+        const unsigned off =
+            pc.take ? 4u * (unsigned)(r.w + l.w + SDGSO_NTYPES * pc.bin) : kNoLoad;
         acc.add(ring[j], true);
         ring[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tbl, off, 0, 0));
         npairs += pc.take ? 1 : 0;

@@ -10,11 +10,13 @@
 //     ltype   i32 [nl]             type * 20
 //     table64 f64 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
 //     table   f32 [168*168*20+20]  the same rounded to FP32 (FP32 guard form)
+//     tablep  f32 [168*3528+64]    the FP32 values as [a][bin 0..20][b] (FP32 form)
+//     ltype_b i32 [nl]             type b (the FP32 form's offset into a tablep row)
 //   receptor tiles (FP32 guard form): 64 atoms per tile, grouped by bisection
 //     tiles   f32x4 [ntiles]       tile centre (x,y,z) and radius
 //     rrel    f64 [3][ntiles*64]   coordinates - tile centre
 //     rmod32  f32 [nmr][3][ntiles*64]
-//     rtype_s i32 [ntiles*64]      type*3360 in tile order
+//     rtype_s i32 [ntiles*64]      type*3528 (a tablep row) in tile order
 //     rperm   i32 [ntiles*64]      original atom index (-1 = padding)
 //   ligand blocks (FP32 guard form): 64 atoms per block, grouped by bisection
 //     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
@@ -38,6 +40,15 @@
 namespace sd {
 
 constexpr int kTableLen = SDGSO_TABLE_LEN + SDGSO_NBINS;
+// The FP32 kernels' copy of the table, [a][bin][b]: entry a*kPRow + bin*168 +
+// b holds the reference's flat entry a*3360 + b*20 + bin, for bin 0..20 (bin
+// 20 is the reference's read past a row at d2 == 225, interface.h:35), and
+// zeros follow.  With the ligand type fastest, the pairs of one receptor atom
+// in one distance shell with ligand atoms of nearby types share 128-byte
+// lines, which one gather instruction fetches once.
+constexpr int kPRow = (SDGSO_NBINS + 1) * SDGSO_NTYPES;           // 3528
+constexpr int kTableLenP = SDGSO_NTYPES * kPRow + 64;             // + zeros
+constexpr int kZeroIdxP = SDGSO_NTYPES * kPRow;                   // a zero
 constexpr int kTile = 64;              // receptor atoms per tile = wavefront width
 constexpr int kBlock = 64;             // ligand atoms per block = wavefront width
 constexpr double kQuantScale = 1048576.0;  // 2^20: fixed-point units per A
@@ -61,8 +72,10 @@ struct DevComplex {
     const double *lmodes;
     const int *rtype;
     const int *ltype;
-    const float *table;      // FP32 copy (FP32 guard form)
+    const float *table;      // FP32 copy (FP32 form, its guard-band pairs)
     const double *table64;   // the caller's FP64 values (FP64 form)
+    const float *tablep;     // FP32 copy in the [a][bin][b] order (FP32 form)
+    const int *ltype_b;      // [nl] ligand type b (offset into tablep rows)
     double lref[3];
     // receptor tiles
     int ntiles;
