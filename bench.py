@@ -405,6 +405,17 @@ def main():
     flop_frac = achieved / FP32_PEAK_TFLOPS
     names = sdgso.tune_variants()
     chosen_names = [names[v] for v in chosen]
+    # per complex: in-cutoff pairs of the last scoring and the tuner's launch
+    # time of its share with the chosen variant
+    per_complex = []
+    g0 = 0
+    for k, (cx, S) in enumerate(zip(wl["complexes"], wl["n_swarms"])):
+        npairs = float(st["pairs"][g0:g0 + S * wl["G"]].sum())
+        g0 += S * wl["G"]
+        ms = tune_ms[chosen_names[k]][k] if tune_ms else None
+        per_complex.append({"atoms": [cx.nr, cx.nl], "swarms": S, "pairs": npairs,
+                            "variant": chosen_names[k], "launch_ms": ms,
+                            "gather_GBps": (4.0 * npairs / (ms * 1e-3) / 1e9) if ms else None})
     # the variant that scores the most work (waves) names the dominant kernel
     blocks = [S * wl["G"] * -(-cx.nl // 64) for cx, S in zip(wl["complexes"], wl["n_swarms"])]
     share = {}
@@ -468,6 +479,7 @@ def main():
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
         "pairs_per_eval": pairs_per_launch / n_glow,
         "filter": filt,
+        "per_complex": per_complex,
         "autotune": {"chosen_per_complex": chosen_names,
                      "ms_per_launch": ({k: sum(v) for k, v in tune_ms.items()} if tune_ms else None)},
     }

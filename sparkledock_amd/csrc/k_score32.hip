@@ -730,7 +730,8 @@ __device__ __forceinline__ T *uniform_ptr(T *p) {
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
-template <int kUnroll, int kMinWaves, bool kMfma, int kWpb, int kTilesB, int kC, int kLB = 1>
+template <int kUnroll, int kMinWaves, bool kMfma, int kWpb, int kTilesB, int kC, int kLB = 1,
+          bool kByComp = false>
 __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArgs a) {
     static_assert(kLB == 1 || (kLB == 2 && !kMfma), "one block, or two with the FP32 pre-test");
     constexpr int kPend = 64 * kTilesB + 64;  // pending near slots: a batch's worth + one round
@@ -994,7 +995,54 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         // in rf, which the packing below overwrites.  The loads of kDispRounds
         // rounds are issued together.
         float4 D = make_float4(0.f, 0.f, 0.f, 0.f);
-        if (nmr > 0) {
+        if constexpr (kByComp) {
+            // component-major modes: float4 j of an atom holds modes
+            // 4*pos .. 4*pos + 3 of coordinate comp (q float4 per coordinate);
+            // each lane weighs its four floats by their extents, and a scan over
+            // the q lanes of a coordinate (shifts 1, 2, 4 within the row, each
+            // lane adding only from lanes of its own coordinate) leaves the
+            // coordinate's sum in its last lane, which stages it in rf
+            const int j = lane & (kC - 1);
+            const int q = (nmr + 3) >> 2;
+            const int comp = (j >= q ? 1 : 0) + (j >= 2 * q ? 1 : 0) + (j >= 3 * q ? 1 : 0);
+            const int pos = j - comp * q;
+            float w4[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int m = 4 * pos + t;
+                w4[t] = comp < 3 && m < nmr ? ef[min(m, SDGSO_MAX_MODES - 1)] : 0.f;
+            }
+            const float g1 = pos >= 1 ? 1.f : 0.f, g2 = pos >= 2 ? 1.f : 0.f,
+                        g4 = pos >= 4 ? 1.f : 0.f;
+            const bool last = comp < 3 && pos == q - 1;
+            constexpr int kApr = 64 / kC;  // atoms per round
+            const int nround = (n + kApr - 1) / kApr;
+            for (int k0 = 0; k0 < nround; k0 += kDispRounds) {
+                float4 qv[kDispRounds];
+                int av[kDispRounds];
+#pragma unroll
+                for (int r = 0; r < kDispRounds; r++) {
+                    av[r] = (k0 + r) * kApr + lane / kC;
+                    qv[r] = (k0 + r < nround && av[r] < n)
+                                ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * kC + j)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (int r = 0; r < kDispRounds; r++) {
+                    if (k0 + r >= nround) break;
+                    float v = qv[r].x * w4[0];
+                    v = __builtin_fmaf(qv[r].y, w4[1], v);
+                    v = __builtin_fmaf(qv[r].z, w4[2], v);
+                    v = __builtin_fmaf(qv[r].w, w4[3], v);
+                    v = __builtin_fmaf(dpp_f32<0x111>(v), g1, v);  // row_shr:1
+                    if (q > 1) v = __builtin_fmaf(dpp_f32<0x112>(v), g2, v);  // row_shr:2
+                    if (q > 4) v = __builtin_fmaf(dpp_f32<0x114>(v), g4, v);  // row_shr:4
+                    if (last && av[r] < n) ((float *)&rf[av[r]])[comp] = v;
+                }
+            }
+            wave_sync_lds();
+            if (lane < n) D = rf[lane];
+        } else if (nmr > 0) {
             const int j = lane & (kC - 1);
             float wx[4], wy[4], wz[4];
 #pragma unroll
@@ -1330,17 +1378,20 @@ __global__ void advance_kernel(int *k) {
 namespace {
 struct Variant {
     const char *name;
-    void (*kernel[3])(ScoreArgs);  // for 8, 16 and 32 float4 of modes per receptor slot
+    // for 8, 16 and 32 float4 of modes per receptor slot, mode-major, and for
+    // 16 component-major (modes_by_component)
+    void (*kernel[4])(ScoreArgs);
     int waves;                     // waves per workgroup
     int blocks = 1;                // ligand blocks per wave (work list of block pairs for 2)
 };
 #define SD_BCAST(R, O, M, T) \
-    { score32b_kernel<R, O, M, 1, T, 8>, score32b_kernel<R, O, M, 1, T, 16>, score32b_kernel<R, O, M, 1, T, 32> }
+    { score32b_kernel<R, O, M, 1, T, 8>, score32b_kernel<R, O, M, 1, T, 16>, \
+      score32b_kernel<R, O, M, 1, T, 32>, score32b_kernel<R, O, M, 1, T, 16, 1, true> }
 #define SD_BCAST2(R, O, T) \
     { score32b_kernel<R, O, false, 1, T, 8, 2>, score32b_kernel<R, O, false, 1, T, 16, 2>, \
-      score32b_kernel<R, O, false, 1, T, 32, 2> }
+      score32b_kernel<R, O, false, 1, T, 32, 2>, score32b_kernel<R, O, false, 1, T, 16, 2, true> }
 #define SD_SAME(K) \
-    { K, K, K }
+    { K, K, K, K }
 const Variant kVariants[] = {
     {"bcast-mfma16/ring3/occ4", SD_BCAST(3, 4, true, 4), 1},
     {"bcast-mfma16/ring2/occ4", SD_BCAST(2, 4, true, 4), 1},
@@ -1380,7 +1431,8 @@ hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     constexpr int kChunk = (1 << 26) - kWaves;
     const int wpb = kVariants[variant].waves;
     const int kc = mode_chunks(a.nmr);
-    void (*kernel)(ScoreArgs) = kVariants[variant].kernel[kc == 8 ? 0 : (kc == 16 ? 1 : 2)];
+    const int ki = modes_by_component(a.nmr) ? 3 : (kc == 8 ? 0 : (kc == 16 ? 1 : 2));
+    void (*kernel)(ScoreArgs) = kVariants[variant].kernel[ki];
     for (int base = 0; base < a.n_work; base += kChunk) {
         ScoreArgs part = a;
         part.work = a.work + 2 * (size_t)base;

@@ -63,6 +63,13 @@ inline __host__ __device__ int mode_chunks(int nm) {
     return lines <= 1 ? 8 : (lines <= 2 ? 16 : 32);
 }
 
+// Order of a slot's FP32 modes in rmodA.  For 11 to 20 modes (16 float4 per
+// slot) component-major, [x of modes 0..nm-1 | y | z] with each coordinate
+// in ceil(nm / 4) whole float4 (zeros past nm): every float4 then holds modes
+// of one coordinate, and the FP32 kernels take a variant for that order.
+// Otherwise mode-major: x, y, z of mode 0, then of mode 1, and so on.
+inline __host__ __device__ bool modes_by_component(int nm) { return nm >= 11 && nm <= 20; }
+
 // Per-complex device descriptor (plain data; lives in a device array).
 struct DevComplex {
     int nr, nl, nmr, nml;
@@ -84,8 +91,9 @@ struct DevComplex {
     const float *rmod32;     // [nmr][3][ntiles*64] modes (f32)
     // the same per slot, for gathers by slot: near-test coordinates (f32,
     // absolute, w = 0), coordinates and type row (f64 absolute x, y, z and
-    // the row rtype_s as a double) and modes (f32, mode-major x, y, z,
-    // padded to 4 * mode_chunks(nmr) = rmod_stride floats)
+    // the row rtype_s as a double) and modes (f32, in the order
+    // modes_by_component(nmr) gives, padded to 4 * mode_chunks(nmr) =
+    // rmod_stride floats)
     const float4 *rpos32;
     const double *rrel4;
     const float *rmodA;

@@ -62,11 +62,14 @@ def test_score_fp64_bit_exact(seed, nmr, nml):
 
 
 @pytest.mark.parametrize("seed,nmr,nml", [(4, 10, 10), (5, 0, 0), (6, 20, 20), (9, 24, 6), (10, 32, 1),
-                                          (11, 1, 0)])
+                                          (11, 1, 0), (12, 8, 3), (13, 11, 2), (14, 16, 5),
+                                          (15, 21, 0), (16, 9, 9)])
 def test_score_fp32_guard(seed, nmr, nml):
     """Receptor mode counts cover every mode-line width of the broadcast
-    variant's cooperative displacement: 8 float4 (up to 10 modes), 16 (up to
-    21) and 32 (up to 32), and none."""
+    variant's cooperative displacement, 8 float4 (up to 10 modes), 16 (up to
+    21) and 32 (up to 32), and none, and both orders of a slot's modes:
+    component-major with 3, 4 and 5 float4 per coordinate (11, 16, 20
+    modes) and mode-major (1, 8, 9, 10, 21, 24, 32)."""
     c = GC.case(seed, 700, 300, nmr, nml, 4, 16)
     pr = GC.Pair(c)
     poses = poses_of(c)
@@ -275,10 +278,13 @@ def large_displacement_case(seed=19, nm=10, ext=25.0):
     return c
 
 
-def test_bins_exact_with_large_displacements():
+@pytest.mark.parametrize("nm", [10, 20, 5])
+def test_bins_exact_with_large_displacements(nm):
     """Each wave widens its guard band to its displacement bound, so bins
-    stay exact when the FP32 mode sum is off by more than the default band."""
-    c = large_displacement_case()
+    stay exact when the FP32 mode sum is off by more than the default band;
+    10 and 5 modes take the mode-major displacement, 20 the component-major
+    one."""
+    c = large_displacement_case(nm=nm)
     pr = GC.Pair(c)
     poses = poses_of(c)
     s0, _, p0 = pr.oracle_scores(poses)
