@@ -249,6 +249,9 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--anm-probe", action="store_true",
+                    help="also time the receptor displacement as an FP32 matrix-core GEMM "
+                         "(sdgso_probe_anm_gemm) on the timed run's poses")
     ap.add_argument("--no-c0", action="store_true",
                     help="skip the full c0 runs (CPU oracle and both GPU forms)")
     ap.add_argument("--ranking-steps", type=int, default=30,
@@ -347,6 +350,13 @@ def main():
     dt1 = timed_steps(run, cmp_steps, graph=True)
     dte = timed_steps(run, cmp_steps, graph=False)
     filt = run.filter_stats()
+    anm = None
+    if args.anm_probe:
+        anm = run.probe_anm_gemm()
+        anm["fused_displacements"] = filt["near_atoms"]
+        anm["gemm_TFLOPs"] = anm["flops"] / (anm["ms"] * 1e-3) / 1e12
+        anm["write_GBps"] = anm["bytes_written"] / (anm["ms"] * 1e-3) / 1e9
+        log("anm gemm probe:", anm)
     log("fp32 guard: %.2f ms/step (profiled graph replays: scoring %.2f ms + rest %.2f ms); "
         "one-graph replays %.2f ms/step, eager %.2f ms/step; pairs/launch %.3e, guard "
         "recomputes %d" % (1e3 * dt / args.steps, score_ms, rest_ms, 1e3 * dt1 / cmp_steps,
@@ -422,8 +432,7 @@ def main():
     for v, w in zip(chosen_names, blocks):
         share[v] = share.get(v, 0) + w
     variant_name = max(share, key=share.get)
-    kernel_name = "score32%s_kernel" % {"bcast": "b", "bcast2": "b", "readlane": "r", "mfma": "m"}.get(
-        variant_name.split("/")[0].split("-")[0], "")
+    kernel_name = "score32b_kernel"  # every FP32 variant is a form of it
     out = {
         "metric": METRIC,
         "value": value,
@@ -480,6 +489,7 @@ def main():
         "pairs_per_eval": pairs_per_launch / n_glow,
         "filter": filt,
         "per_complex": per_complex,
+        "anm_gemm_probe": anm,
         "autotune": {"chosen_per_complex": chosen_names,
                      "ms_per_launch": ({k: sum(v) for k, v in tune_ms.items()} if tune_ms else None)},
     }

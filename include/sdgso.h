@@ -171,9 +171,12 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
  * reference, whose tile sizes are compile-time constants,
  * prepare_gso_task.cu:2503-2514).  sdgso_tune_variants() is the number of
  * compiled variants (ligand-block size x filter form x gathers in flight per
- * lane x register/occupancy target); a name starting with "bcast2/" is a
- * variant whose wave takes a 128-atom ligand block (two 64-atom blocks of one
- * glowworm), the others take 64 atoms.  sdgso_run_tune scores the current poses of
+ * lane x register/occupancy target x work distribution); a name starting
+ * with "bcast2" is a variant whose wave takes a 128-atom ligand block (two
+ * 64-atom blocks of one glowworm), the others take 64 atoms; "-stride" and
+ * "-queue" name the persistent forms (a grid of resident waves taking the
+ * work list by a fixed stride, or 8 items at a time from an atomic counter),
+ * the others one work item per wave.  sdgso_run_tune scores the current poses of
  * each complex of the run with each variant `repeats` times (luciferin and
  * step counters untouched) and keeps, per complex, the fastest for later
  * steps: times_ms[n_complex][sdgso_tune_variants()] receives the
@@ -201,6 +204,16 @@ int sdgso_run_filter_stats(sdgso_run r, int64_t *stats);
  * entries (no counterpart in the reference): the measured ceiling of the
  * scoring kernels' table reads.  Returns gathers per second in *rate. */
 int sdgso_probe_gather(int64_t table_floats, double *rate);
+
+/* The receptor's normal-mode displacement as an FP32 matrix-core GEMM, timed
+ * on the run's current poses (no counterpart in the reference, a measurement
+ * of the form the FP32 kernels do not take): for every swarm, the
+ * displacement of every receptor atom in the tiles that some glowworm of the
+ * swarm can reach, for each of its glowworms, by v_mfma_f32_32x32x2f32, and
+ * written to HBM.  result[5] = {milliseconds, bytes written, matrix-core
+ * flops issued, displacements computed, (swarm, tile) pairs}.  The run must
+ * be in the FP32 form and have receptor modes; its state is unchanged. */
+int sdgso_probe_anm_gemm(sdgso_run r, double *result);
 
 /* ---- output side of a run ---- */
 

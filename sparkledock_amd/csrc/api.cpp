@@ -182,8 +182,10 @@ using namespace sd;
 // ---------------------------------------------------------------- complex
 struct sdgso_complex_s {
     int nr = 0, nl = 0, nmr = 0, nml = 0;
-    // host copies kept for argument checks and statistics
-    std::vector<double> r_xyz, l_xyz;
+    // host copies for the measurement probes: receptor tiles (centre,
+    // radius) and the ligand's largest |xyz| about its frame's origin
+    std::vector<float4> h_tiles;
+    double lig_rmax = 0;
     DevBuf<double> r0, rmodes, l0, lmodes;
     DevBuf<int> rtype, ltype;
     DevBuf<double> table64;    // the caller's table (FP64 form)
@@ -191,13 +193,12 @@ struct sdgso_complex_s {
     DevBuf<float> tablep;      // the FP32 values as [a][bin][b] (FP32 form's gathers)
     DevBuf<int> ltype_b;       // ligand types
     DevBuf<float4> tiles, lblocks, lhalf;
-    DevBuf<double> rrel;       // [3][ntiles*64] receptor coordinates - tile centre, f64
     DevBuf<float> rmod32;      // [nmr][3][ntiles*64]
     DevBuf<float4> rpos32;     // [ntiles*64] f32 coordinates (near test)
     DevBuf<double> rrel4;      // [ntiles*64][4] f64 absolute coordinates, type row
     DevBuf<float> rmodA;       // [ntiles*64][rmod_stride] modes by slot
     int rmod_stride = 0;
-    DevBuf<int> rtype_s, rperm, lperm;
+    DevBuf<int> rperm, lperm;
     DevBuf<double> lblk;       // [nlblocks][3][64] ligand coordinates by block slot
     DevBuf<float> lblkm;       // [nlblocks][3*nml][64] ligand modes by block slot, FP32
     DevComplex dev{};          // descriptor with device pointers
@@ -223,6 +224,7 @@ struct sdgso_run_s {
     DevBuf<long long> partial_pairs;
     DevBuf<int> d_k;
     DevBuf<unsigned long long> guard_count;
+    DevBuf<unsigned> counter;  // the work-queue variants' counter
     hipStream_t own_stream = nullptr;
     hipStream_t stream_h = nullptr;
     hipGraphExec_t graph = nullptr;        // one step
@@ -347,13 +349,12 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
         c->dev.rmode_max[m] = (float)(mx * (1 + 1e-5));
     }
     c->tiles.upload(tiles);
-    c->rrel.upload(rrel);
+    c->h_tiles = tiles;
     c->rmod32.upload(m32);
     c->rpos32.upload(p32);
     c->rrel4.upload(r4);
     c->rmodA.upload(mA);
     c->rmod_stride = stride;
-    c->rtype_s.upload(ts);
     c->rperm.upload(perm);
     c->dev.ntiles = ntiles;
 }
@@ -522,6 +523,10 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.nl = n_lig;
         d.nmr = n_rec_modes;
         d.nml = n_lig_modes;
+        for (int i = 0; i < n_lig; i++)
+            c->lig_rmax = std::max(c->lig_rmax, std::sqrt(lig_xyz[3 * (size_t)i] * lig_xyz[3 * (size_t)i] +
+                                                          lig_xyz[3 * (size_t)i + 1] * lig_xyz[3 * (size_t)i + 1] +
+                                                          lig_xyz[3 * (size_t)i + 2] * lig_xyz[3 * (size_t)i + 2]));
         build_tiles(c.get(), rec_xyz, rec_type, rmsoa);
         build_lblocks(c.get(), lig_xyz, lig_type, lmsoa);
         d.r0 = c->r0.p;
@@ -541,14 +546,12 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.rrel4 = c->rrel4.p;
         d.rmodA = c->rmodA.p;
         d.rmod_stride = c->rmod_stride;
-        d.rtype_s = c->rtype_s.p;
         d.rperm = c->rperm.p;
         d.lperm = c->lperm.p;
         d.lblocks = c->lblocks.p;
         d.lhalf = c->lhalf.p;
         d.lblk = c->lblk.p;
         d.lblkm = c->lblkm.p;
-        d.rrel = c->rrel.p;
         c->dself.upload(&d, 1);
         *out = c.release();
     });
@@ -597,6 +600,7 @@ static ScoreArgs score_args(sdgso_run r) {
     a.nmr = r->nmr;
     a.work = r->work.p;
     a.n_work = (int)r->work.n;
+    a.counter = r->counter.p;
     return a;
 }
 
@@ -873,6 +877,7 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
             r->work2.upload(work2);
             r->work2.n = work2.size() / 2;
             r->variants.assign(n_complex, score32_default_variant());
+            r->counter.alloc(1);
         }
         HIPCHK(hipStreamCreateWithFlags(&r->own_stream, hipStreamNonBlocking));
         r->stream_h = r->own_stream;
@@ -1179,6 +1184,87 @@ int sdgso_probe_gather(int64_t table_floats, double *rate) {
         (void)hipEventDestroy(e1);
         (void)hipStreamDestroy(s);
         *rate = (double)blocks * 256 * per_thread / (ms * 1e-3);
+    });
+}
+
+int sdgso_probe_anm_gemm(sdgso_run r, double *result) {
+    return guarded([&] {
+        if (!r || !result) fail(SDGSO_EINVAL, "null argument");
+        if (r->form != SDGSO_FP32_GUARD) fail(SDGSO_EINVAL, "the probe needs a run in the FP32 form");
+        if (r->nmr <= 0) fail(SDGSO_EINVAL, "the probe needs receptor modes");
+        ensure_device();
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        const int G = r->G, P = r->P;
+        std::vector<double> pos((size_t)r->N * P);
+        HIPCHK(hipMemcpy(pos.data(), r->pos.p, pos.size() * sizeof(double), hipMemcpyDeviceToHost));
+        // per swarm: the tiles any of its glowworms can reach (translation
+        // spread + the ligand's extent + its and the receptor's mode
+        // displacement bounds + 15 A + the tile's radius)
+        std::vector<int> pairs;  // (swarm, complex, tile)
+        int s0 = 0;
+        for (int k = 0; k < r->n_complex; k++) {
+            const sdgso_complex c = r->cxs[k];
+            for (int s = s0; s < s0 + r->n_swarms[k]; s++) {
+                double cen[3] = {0, 0, 0}, spread = 0, dl = 0, dr = 0;
+                for (int g = 0; g < G; g++)
+                    for (int d = 0; d < 3; d++) cen[d] += pos[((size_t)s * G + g) * P + d] / G;
+                for (int g = 0; g < G; g++) {
+                    const double *q = pos.data() + ((size_t)s * G + g) * P;
+                    double t2 = 0, er = 0, el = 0;
+                    for (int d = 0; d < 3; d++) t2 += (q[d] - cen[d]) * (q[d] - cen[d]);
+                    for (int m = 0; m < c->nmr; m++) er += std::fabs(q[7 + m]) * c->dev.rmode_max[m];
+                    for (int m = 0; m < c->nml; m++) el += std::fabs(q[7 + c->nmr + m]) * c->dev.lmode_max[m];
+                    spread = std::max(spread, std::sqrt(t2));
+                    dr = std::max(dr, er);
+                    dl = std::max(dl, el);
+                }
+                const double reach = spread + c->lig_rmax + dl + 15.0 + dr + 0.01;
+                for (int t = 0; t < (int)c->h_tiles.size(); t++) {
+                    const float4 tl = c->h_tiles[t];
+                    const double dx = tl.x - cen[0], dy = tl.y - cen[1], dz = tl.z - cen[2];
+                    const double lim = reach + tl.w;
+                    if (dx * dx + dy * dy + dz * dz <= lim * lim) {
+                        pairs.push_back(s);
+                        pairs.push_back(k);
+                        pairs.push_back(t);
+                    }
+                }
+            }
+            s0 += r->n_swarms[k];
+        }
+        const size_t npairs = pairs.size() / 3;
+        const int chunks = (G + 31) / 32;
+        // D of a (swarm, tile, 32-glowworm chunk): 32 x 192 floats; written to
+        // a buffer of at most 4 GiB that later launches overwrite
+        const size_t per_item = (size_t)32 * 192 * sizeof(float);
+        const size_t items = npairs * chunks;
+        const size_t cap = std::max<size_t>(1, std::min<size_t>(items, ((size_t)4 << 30) / per_item));
+        DevBuf<int> dpairs;
+        dpairs.upload(pairs);
+        DevBuf<float> out;
+        out.alloc(cap * 32 * 192);
+        hipEvent_t e0, e1;
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        // warm-up on the first items, then the timed pass over all of them
+        HIPCHK(launch_anm_gemm_probe(r->dcx.p, r->pos.p, P, G, dpairs.p, 0, std::min(items, cap),
+                                     chunks, out.p, r->stream_h));
+        HIPCHK(hipEventRecord(e0, r->stream_h));
+        for (size_t i0 = 0; i0 < items; i0 += cap)
+            HIPCHK(launch_anm_gemm_probe(r->dcx.p, r->pos.p, P, G, dpairs.p, i0, std::min(cap, items - i0),
+                                         chunks, out.p, r->stream_h));
+        HIPCHK(hipEventRecord(e1, r->stream_h));
+        HIPCHK(hipEventSynchronize(e1));
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        const int kpad = (r->nmr + 1) / 2 * 2;
+        result[0] = ms;
+        result[1] = (double)items * per_item;                      // bytes written
+        result[2] = 2.0 * kpad * 192.0 * 32.0 * (double)items;     // MFMA flops issued
+        result[3] = (double)npairs * 64.0 * G;                     // displacements computed
+        result[4] = (double)npairs;                                // (swarm, tile) pairs
     });
 }
 

@@ -15,18 +15,14 @@
 // receptor atoms that can reach the block are displaced (FP32 modes), rounded
 // to the same grid and pre-tested against the block's ligand atoms.
 //
-// Kernels (the run's autotuner picks one per complex):
-//   score32b ("bcast", "bcast2"): near atoms are batched, displaced by
-//     cooperative lanes, box-tested, packed into LDS and pre-tested with
-//     broadcast reads (FP32 or half-precision MFMA); passing pairs are queued
-//     as one word and evaluated 64 per round (described at the kernel);
-//   score32r ("readlane"): kept receptor atoms are broadcast one at a time
-//     and every lane forms the exact fixed-point d2 with its ligand atom;
-//   score32m ("mfma"): d2 = |r|^2 + |l|^2 - 2 r.l of 32 x 32 pairs by three
-//     v_mfma_f32_32x32x2f32 (r' = (x, y, z, |r|^2, 1, 0), l' = (-2x, -2y, -2z,
-//     1, |l|^2, 0)); its FP32 cancellation error is below 2e-3 A^2 in the
-//     block frame, so pairs with d2 <= 225 + 2 band + 0.05 are queued and
-//     evaluated exactly.
+// Kernel: score32b; near atoms are batched, displaced by cooperative lanes,
+// box-tested, packed into LDS and pre-tested with broadcast reads (FP32 or
+// half-precision MFMA); passing pairs are queued as one word and evaluated
+// 64 per round (described at the kernel).  The run's autotuner picks a
+// variant per complex: one or two ligand blocks per wave ("bcast",
+// "bcast2"), the pre-test form, and how waves get their work items (one item
+// per wave from the hardware dispatcher, or a grid of resident waves taking
+// items from a queue, "-queue").
 //
 // Evaluate (all 64 lanes busy): exact d2 from the fixed-point coordinates
 // (integer differences are exact and convert to FP32 exactly below 16 A), the
@@ -55,10 +51,6 @@ namespace sd {
 typedef float floatx16 __attribute__((ext_vector_type(16)));
 typedef _Float16 half4 __attribute__((ext_vector_type(4)));
 
-constexpr int kScore32Threads = 256;
-constexpr int kWaves = kScore32Threads / 64;
-constexpr int kQueueR = 256;                  // readlane variant: queued pairs per wave
-constexpr int kQueueM = 1536;                 // mfma variant: room for one 32x32 group
 constexpr int kMaxTiles = kMaxTiles32;        // receptor tiles per complex (host-checked)
 constexpr float kQ = 1048576.0f;              // 2^20 units per A
 constexpr float kInvQ = 1.0f / 1048576.0f;
@@ -87,16 +79,43 @@ struct CompSum {
     }
 };
 
+// x/y/z += M[m][.][i] * e_m for m in mode order, one mode at a time (the
+// reference order of prepare_gso_tasks.cc:465-504, as modes_add64).
+__device__ __forceinline__ void modes_add64_lean(const double *modes, int n, int i,
+                                                 const double *ext, int nm, double &x, double &y,
+                                                 double &z) {
+#pragma unroll 1
+    for (int m = 0; m < nm; m++) {
+        const double *M = modes + (size_t)m * 3 * n;
+        const double e = ext[m];
+        x += gld(M, i) * e;
+        y += gld(M, n + i) * e;
+        z += gld(M, 2 * n + i) * e;
+    }
+}
+
 // FP64 decision for a pair inside the guard band of a boundary, rebuilt from
-// the pose: returns (table value, 1) for a pair inside (0, 225], else (0, 0).
+// the pose in the reference's order: returns (table value, 1) for a pair
+// inside (0, 225], else (0, 0).  Called from the evaluation loop of the FP32
+// kernels for about 2e-5 of the pairs, so it is kept out of line and lean:
+// its mode sums run one mode at a time, which leaves most of the caller's
+// registers untouched across the call.
 __device__ __noinline__ float2 guard_pair64(const DevComplex &c, const double *pose,
                                             int lig_atom, int rec_atom) {
     if (lig_atom < 0 || rec_atom < 0) return make_float2(0.f, 0.f);
+    double Rx = gld(c.r0, rec_atom), Ry = gld(c.r0, c.nr + rec_atom),
+           Rz = gld(c.r0, 2 * c.nr + rec_atom);
+    modes_add64_lean(c.rmodes, c.nr, rec_atom, pose + 7, c.nmr, Rx, Ry, Rz);
+    double Lx = gld(c.l0, lig_atom), Ly = gld(c.l0, c.nl + lig_atom),
+           Lz = gld(c.l0, 2 * c.nl + lig_atom);
+    modes_add64_lean(c.lmodes, c.nl, lig_atom, pose + 7 + c.nmr, c.nml, Lx, Ly, Lz);
     double inv[4];
     quat_inverse(pose + 3, inv);
-    double Rx, Ry, Rz, Lx, Ly, Lz, v64 = 0.0;
-    receptor_atom64(c, pose + 7, rec_atom, Rx, Ry, Rz);
-    ligand_atom64(c, pose, inv, lig_atom, Lx, Ly, Lz);
+    rotate_point(pose + 3, inv, Lx, Ly, Lz);
+    Lx += pose[0];
+    Ly += pose[1];
+    Lz += pose[2];
+    double v64 = 0.0;
     const bool hit = pair_term64(Rx, Ry, Rz, Lx, Ly, Lz, gld(c.rtype, rec_atom),
                                  gld(c.ltype, lig_atom), c.table, v64);
     return make_float2(hit ? (float)v64 : 0.f, hit ? 1.f : 0.f);
@@ -252,52 +271,6 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     return true;
 }
 
-// Tile visit: this lane's receptor atom of tile t (rel. coordinates already
-// loaded): near flag and, for near lanes, its displaced fixed-point position
-// in the block frame and table row.
-__device__ __forceinline__ bool tile_atom(const DevComplex &c, const WaveCtx &x, const float *ef,
-                                          int t, float4 tl, double rx, double ry, double rz,
-                                          int4 &R) {
-    const int slots = c.ntiles * kTile;
-    const int slot = t * kTile + x.lane;
-    const float ax = (float)rx + (tl.x - x.fc[0]);
-    const float ay = (float)ry + (tl.y - x.fc[1]);
-    const float az = (float)rz + (tl.z - x.fc[2]);
-    const bool nearb = ax * ax + ay * ay + az * az <= x.reach * x.reach;
-    R = make_int4(kFar, kFar, kFar, 0);
-    if (nearb) {
-        float Dx = 0.f, Dy = 0.f, Dz = 0.f;
-        int m = 0;
-        for (; m + 4 <= c.nmr; m += 4) {
-            float mx[4], my[4], mz[4];
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                const float *M = c.rmod32 + (size_t)(m + k) * 3 * slots;
-                mx[k] = gld(M, slot);
-                my[k] = gld(M, slots + slot);
-                mz[k] = gld(M, 2 * slots + slot);
-            }
-#pragma unroll
-            for (int k = 0; k < 4; k++) {
-                Dx = __builtin_fmaf(mx[k], ef[m + k], Dx);
-                Dy = __builtin_fmaf(my[k], ef[m + k], Dy);
-                Dz = __builtin_fmaf(mz[k], ef[m + k], Dz);
-            }
-        }
-        for (; m < c.nmr; m++) {
-            const float *M = c.rmod32 + (size_t)m * 3 * slots;
-            Dx = __builtin_fmaf(gld(M, slot), ef[m], Dx);
-            Dy = __builtin_fmaf(gld(M, slots + slot), ef[m], Dy);
-            Dz = __builtin_fmaf(gld(M, 2 * slots + slot), ef[m], Dz);
-        }
-        R = make_int4(qround(((double)tl.x - x.cb[0]) + rx + (double)Dx),
-                      qround(((double)tl.y - x.cb[1]) + ry + (double)Dy),
-                      qround(((double)tl.z - x.cb[2]) + rz + (double)Dz),
-                      gld(c.rtype_s, slot));
-    }
-    return nearb;
-}
-
 // The wave's energy and pair count go to block x.b's partial; a second block
 // b2 (>= 0) of a two-block wave gets zeros.
 __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
@@ -332,297 +305,6 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
             atomicAdd(st + 5 * kStatSlots, (unsigned long long)np);
         }
     }
-}
-
-// ---------------------------------------------------------------- readlane variant
-template <int kUnroll, int kMinWaves>
-__global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32r_kernel(ScoreArgs a) {
-    __shared__ float s_d2[kWaves][kQueueR];   // queued pairs: fixed-point d2
-    __shared__ int s_row[kWaves][kQueueR];    //   tablep offset a*kPRow + b
-    __shared__ int s_tag[kWaves][kQueueR];    //   (ligand lane << 24) | receptor atom
-    __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
-    __shared__ unsigned short s_ct[kWaves][kMaxTiles];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wv = blockIdx.x * kWaves + w;
-    if (wv >= a.n_work) return;
-    const int gid = a.work[2 * wv];
-    const int b = a.work[2 * wv + 1];
-    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
-    float *ef = s_ef[w];
-    unsigned short *ct = s_ct[w];
-    float *qd = s_d2[w];
-    int *qr = s_row[w];
-    int *qt = s_tag[w];
-    WaveCtx x;
-    wave_setup(a, c, gid, b, ef, ct, x);
-    const int lane = x.lane;
-    const float bandq = x.band * kQ2;
-    const float cutq = (225.0f + 2.0f * x.band) * kQ2;
-    const int slots = c.ntiles * kTile;
-
-    CompSum acc;
-    int npairs = 0;
-    unsigned int nguard = 0;
-    int qn = 0;
-    long long n_near = 0, n_queued = 0;
-    auto flush = [&]() {
-        wave_sync_lds();
-        n_queued += qn;
-        for (int base = 0; base < qn; base += 64 * kUnroll) {
-            float v[kUnroll];
-            bool take[kUnroll], near[kUnroll];
-            int tag[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const int e = base + u * 64 + lane;
-                const bool valid = e < qn;
-                const int ec = min(e, kQueueR - 1);
-                const float d2q = qd[ec];
-                const int row = qr[ec];
-                tag[u] = qt[ec];
-                const PairClass pc = classify(d2q, bandq);
-                near[u] = valid && pc.near;
-                take[u] = valid && pc.take;
-                v[u] = gld(c.tablep, take[u] ? row + SDGSO_NTYPES * pc.bin : kZeroIdxP);
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                acc.add(v[u], take[u]);
-                npairs += take[u] ? 1 : 0;
-                if (near[u]) {
-                    const float2 g2 = guard_pair64(c, x.pose,
-                                                   gld(c.lperm, x.b * kBlock + (tag[u] >> 24)),
-                                                   tag[u] & 0xffffff);
-                    acc.add(g2.x, g2.y != 0.f);
-                    npairs += g2.y != 0.f ? 1 : 0;
-                    nguard++;
-                }
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        qn = 0;
-    };
-
-    int t_next = 0;
-    float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f);
-    double rx_next = 0, ry_next = 0, rz_next = 0;
-    if (x.ncand) {
-        t_next = __builtin_amdgcn_readfirstlane((int)ct[0]);
-        tl_next = gld(c.tiles, t_next);
-        rx_next = gld(c.rrel, t_next * kTile + lane);
-        ry_next = gld(c.rrel, slots + t_next * kTile + lane);
-        rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
-    }
-    for (int k = 0; k < x.ncand; k++) {
-        const int t = t_next;
-        const float4 tl = tl_next;
-        const double rx = rx_next, ry = ry_next, rz = rz_next;
-        if (k + 1 < x.ncand) {
-            t_next = __builtin_amdgcn_readfirstlane((int)ct[k + 1]);
-            tl_next = gld(c.tiles, t_next);
-            rx_next = gld(c.rrel, t_next * kTile + lane);
-            ry_next = gld(c.rrel, slots + t_next * kTile + lane);
-            rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
-        }
-        int4 R;
-        const bool nearb = tile_atom(c, x, ef, t, tl, rx, ry, rz, R);
-        unsigned long long mask = __ballot(nearb);
-        n_near += __popcll(mask);
-        const int rorig = nearb ? gld(c.rperm, t * kTile + lane) : 0;
-        const int tag_l = lane << 24;
-        while (mask) {
-            const int i = __ffsll((long long)mask) - 1;
-            mask &= mask - 1;
-            const int4 Ri = make_int4(__builtin_amdgcn_readlane(R.x, i),
-                                      __builtin_amdgcn_readlane(R.y, i),
-                                      __builtin_amdgcn_readlane(R.z, i), 0);
-            const int sa = __builtin_amdgcn_readlane(R.w, i);
-            const float d2q = d2_fixed(x.L, Ri);
-            const bool in = d2q <= cutq;
-            const unsigned long long m = __ballot(in);
-            if (m) {
-                if (in) {
-                    const int pos = qn + __builtin_amdgcn_mbcnt_hi(
-                                             (unsigned)(m >> 32),
-                                             __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                    qd[pos] = d2q;
-                    qr[pos] = sa + x.L.w;
-                    qt[pos] = tag_l | __builtin_amdgcn_readlane(rorig, i);
-                }
-                qn += __popcll(m);
-                if (qn > kQueueR - 64) flush();
-            }
-        }
-    }
-    if (qn) flush();
-    write_partials(a, x, acc, npairs, nguard, x.ncand, n_near, n_queued);
-}
-
-// ---------------------------------------------------------------- mfma variant
-template <int kUnroll, int kMinWaves>
-__global__ __launch_bounds__(kScore32Threads, kMinWaves) void score32m_kernel(ScoreArgs a) {
-    __shared__ int4 s_rt[kWaves][64];     // packed receptor atoms of the tile: fixed point + row
-    __shared__ int s_ro[kWaves][64];      // their original indices
-    __shared__ int4 s_lg[kWaves][64];     // the block's ligand atoms: fixed point + type*20
-    __shared__ int s_q[kWaves][kQueueM];  // queued pairs: (packed receptor << 8) | ligand lane
-    __shared__ float s_ef[kWaves][SDGSO_MAX_MODES];
-    __shared__ unsigned short s_ct[kWaves][kMaxTiles];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wv = blockIdx.x * kWaves + w;
-    if (wv >= a.n_work) return;
-    const int gid = a.work[2 * wv];
-    const int b = a.work[2 * wv + 1];
-    const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
-    int4 *rt = s_rt[w];
-    int *ro = s_ro[w];
-    int4 *lg = s_lg[w];
-    int *qq = s_q[w];
-    float *ef = s_ef[w];
-    unsigned short *ct = s_ct[w];
-    WaveCtx x;
-    wave_setup(a, c, gid, b, ef, ct, x);
-    const int lane = x.lane;
-    lg[lane] = x.L;
-    const float bandq = x.band * kQ2;
-    const float thr = 225.0f + 2.0f * x.band + kFilterMargin;
-    const int slots = c.ntiles * kTile;
-    const bool lo_half = lane < 32;
-    wave_sync_lds();
-    // B operands of the two ligand sub-blocks: l' = (-2x, -2y, -2z, 1, |l|^2, 0)
-    float bop[2][3];
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-        const int4 q = lg[32 * g + (lane & 31)];
-        const float px = (float)q.x * kInvQ, py = (float)q.y * kInvQ, pz = (float)q.z * kInvQ;
-        const float l2 = px * px + py * py + pz * pz;
-        bop[g][0] = lo_half ? -2.f * px : -2.f * py;
-        bop[g][1] = lo_half ? -2.f * pz : 1.f;
-        bop[g][2] = lo_half ? l2 : 0.f;
-    }
-
-    CompSum acc;
-    int npairs = 0;
-    unsigned int nguard = 0;
-    int qn = 0;
-    long long n_near = 0, n_queued = 0;
-    auto flush = [&]() {
-        wave_sync_lds();
-        n_queued += qn;
-        for (int base = 0; base < qn; base += 64 * kUnroll) {
-            float v[kUnroll];
-            bool take[kUnroll], near[kUnroll];
-            int tag[kUnroll];
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                const int e = base + u * 64 + lane;
-                const bool valid = e < qn;
-                const int raw = qq[min(e, kQueueM - 1)];
-                tag[u] = valid ? raw : 0;
-                const int4 r = rt[tag[u] >> 8];
-                const int4 l = lg[tag[u] & 63];
-                const PairClass pc = classify(d2_fixed(l, r), bandq);
-                near[u] = valid && pc.near;
-                take[u] = valid && pc.take;
-                v[u] = gld(c.tablep, take[u] ? r.w + l.w + SDGSO_NTYPES * pc.bin : kZeroIdxP);
-            }
-#pragma unroll
-            for (int u = 0; u < kUnroll; u++) {
-                acc.add(v[u], take[u]);
-                npairs += take[u] ? 1 : 0;
-                if (near[u]) {
-                    const float2 g2 = guard_pair64(c, x.pose,
-                                                   gld(c.lperm, x.b * kBlock + (tag[u] & 63)),
-                                                   ro[tag[u] >> 8]);
-                    acc.add(g2.x, g2.y != 0.f);
-                    npairs += g2.y != 0.f ? 1 : 0;
-                    nguard++;
-                }
-            }
-        }
-        __builtin_amdgcn_wave_barrier();
-        qn = 0;
-    };
-
-    int t_next = 0;
-    float4 tl_next = make_float4(0.f, 0.f, 0.f, 0.f);
-    double rx_next = 0, ry_next = 0, rz_next = 0;
-    if (x.ncand) {
-        t_next = __builtin_amdgcn_readfirstlane((int)ct[0]);
-        tl_next = gld(c.tiles, t_next);
-        rx_next = gld(c.rrel, t_next * kTile + lane);
-        ry_next = gld(c.rrel, slots + t_next * kTile + lane);
-        rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
-    }
-    for (int k = 0; k < x.ncand; k++) {
-        const int t = t_next;
-        const float4 tl = tl_next;
-        const double rx = rx_next, ry = ry_next, rz = rz_next;
-        if (k + 1 < x.ncand) {
-            t_next = __builtin_amdgcn_readfirstlane((int)ct[k + 1]);
-            tl_next = gld(c.tiles, t_next);
-            rx_next = gld(c.rrel, t_next * kTile + lane);
-            ry_next = gld(c.rrel, slots + t_next * kTile + lane);
-            rz_next = gld(c.rrel, 2 * slots + t_next * kTile + lane);
-        }
-        int4 R;
-        const bool nearb = tile_atom(c, x, ef, t, tl, rx, ry, rz, R);
-        const unsigned long long nmask = __ballot(nearb);
-        if (!nmask) continue;
-        const int n_near_t = __popcll(nmask);
-        n_near += n_near_t;
-        if (nearb) {
-            const int pos = __builtin_amdgcn_mbcnt_hi((unsigned)(nmask >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo((unsigned)nmask, 0u));
-            rt[pos] = R;
-            ro[pos] = gld(c.rperm, t * kTile + lane);
-        }
-        wave_sync_lds();
-        for (int h = 0; h * 32 < n_near_t; h++) {
-            // A operands of 32 packed receptor atoms: r' = (x, y, z, |r|^2, 1, 0)
-            const int rc = 32 * h + (lane & 31);
-            const int4 q = rc < n_near_t ? rt[rc] : make_int4(kFar, kFar, kFar, 0);
-            const float px = (float)q.x * kInvQ, py = (float)q.y * kInvQ, pz = (float)q.z * kInvQ;
-            const float r2 = px * px + py * py + pz * pz;
-            const float a0 = lo_half ? px : py;
-            const float a1 = lo_half ? pz : r2;
-            const float a2 = lo_half ? 1.f : 0.f;
-            // row of accumulator v in lane l: 8*(v/4) + 4*(l/32) + v%4 (column l%32)
-            const int rowbase = 32 * h + 4 * (lane >> 5);
-#pragma unroll 1
-            for (int g = 0; g < 2; g++) {
-                if (32 * g >= x.nblk) break;
-                const float b0 = g ? bop[1][0] : bop[0][0];
-                const float b1 = g ? bop[1][1] : bop[0][1];
-                const float b2 = g ? bop[1][2] : bop[0][2];
-                floatx16 d2 = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f,
-                               0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                d2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a0, b0, d2, 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a1, b1, d2, 0, 0, 0);
-                d2 = __builtin_amdgcn_mfma_f32_32x32x2f32(a2, b2, d2, 0, 0, 0);
-                const int col = 32 * g + (lane & 31);
-                // the queue has room for a whole 32 x 32 group (qn <= kQueueM - 1024)
-#pragma unroll
-                for (int v = 0; v < 16; v++) {
-                    const bool in = d2[v] <= thr;
-                    const unsigned long long m = __ballot(in);
-                    if (m) {
-                        if (in) {
-                            const int pos = qn + __builtin_amdgcn_mbcnt_hi(
-                                                     (unsigned)(m >> 32),
-                                                     __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
-                            qq[pos] = ((rowbase + 8 * (v >> 2) + (v & 3)) << 8) | col;
-                        }
-                        qn += __popcll(m);
-                    }
-                }
-                // flush when the next group might not fit, and before the packed
-                // tile in LDS is replaced by the next tile
-                const bool last = 32 * (h + 1) >= n_near_t && 32 * (g + 1) >= x.nblk;
-                if (qn > kQueueM - 1024 || (last && qn)) flush();
-            }
-        }
-    }
-    write_partials(a, x, acc, npairs, nguard, x.ncand, n_near, n_queued);
 }
 
 // ---------------------------------------------------------------- broadcast variant
@@ -730,8 +412,15 @@ __device__ __forceinline__ T *uniform_ptr(T *p) {
     return (T *)(((unsigned long long)hi << 32) | lo);
 }
 
+// Work items per atomic of the work-queue form (kPersist = 2).
+constexpr int kQChunk = 8;
+
+// kPersist: 0, one work item per wave (the hardware dispatcher balances the
+// waves); 1, a grid of resident waves, wave k taking items k, k + W, k + 2W,
+// ... (W waves); 2, a grid of resident waves taking kQChunk consecutive items
+// per atomic from a counter that starts at zero.
 template <int kUnroll, int kMinWaves, bool kMfma, int kWpb, int kTilesB, int kC, int kLB = 1,
-          bool kByComp = false>
+          bool kByComp = false, int kPersist = 0>
 __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArgs a) {
     static_assert(kLB == 1 || (kLB == 2 && !kMfma), "one block, or two with the FP32 pre-test");
     constexpr int kPend = 64 * kTilesB + 64;  // pending near slots: a batch's worth + one round
@@ -749,8 +438,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     __shared__ unsigned short s_ct[kWpb][kCandB];  // candidate tiles, scanned in chunks
     __shared__ float4 s_box[kWpb][5];        // the box test's parameters (read once per batch)
     const int w = kWpb == 1 ? 0 : __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int wv = blockIdx.x * kWpb + w;
-    if (wv >= a.n_work) return;
+    // one work item: (glowworm, first ligand block) wv of the list
+    auto item = [&](const int wv) {
     const int gid = __builtin_amdgcn_readfirstlane(a.work[2 * wv]);
     const int b = __builtin_amdgcn_readfirstlane(a.work[2 * wv + 1]);
     const DevComplex &c = a.cx[__builtin_amdgcn_readfirstlane(a.swarm_cx[gid / a.G])];
@@ -1337,6 +1026,21 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
 #pragma unroll
     for (int j = 0; j < kUnroll; j++) acc.add(ring[j], true);
     write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued, b2);
+    };
+    if constexpr (kPersist == 0) {
+        const int wv = blockIdx.x * kWpb + w;
+        if (wv < a.n_work) item(wv);
+    } else if constexpr (kPersist == 1) {
+        for (int wv = blockIdx.x * kWpb + w; wv < a.n_work; wv += gridDim.x * kWpb) item(wv);
+    } else {
+        for (;;) {
+            unsigned v = 0;
+            if ((threadIdx.x & 63) == 0) v = atomicAdd(a.counter, (unsigned)kQChunk);
+            const int q0 = (int)__builtin_amdgcn_readfirstlane(v);
+            if (q0 >= a.n_work) return;
+            for (int wv = q0; wv < min(q0 + kQChunk, a.n_work); wv++) item(wv);
+        }
+    }
 }
 
 // Per glowworm: energy = partials in block order (FP64), score, luciferin,
@@ -1383,6 +1087,7 @@ struct Variant {
     void (*kernel[4])(ScoreArgs);
     int waves;                     // waves per workgroup
     int blocks = 1;                // ligand blocks per wave (work list of block pairs for 2)
+    int persist = 0;               // kPersist of the kernel: a grid of resident waves if > 0
 };
 #define SD_BCAST(R, O, M, T) \
     { score32b_kernel<R, O, M, 1, T, 8>, score32b_kernel<R, O, M, 1, T, 16>, \
@@ -1390,18 +1095,15 @@ struct Variant {
 #define SD_BCAST2(R, O, T) \
     { score32b_kernel<R, O, false, 1, T, 8, 2>, score32b_kernel<R, O, false, 1, T, 16, 2>, \
       score32b_kernel<R, O, false, 1, T, 32, 2>, score32b_kernel<R, O, false, 1, T, 16, 2, true> }
-#define SD_SAME(K) \
-    { K, K, K, K }
+#define SD_BCAST2P(R, O, T, P) \
+    { score32b_kernel<R, O, false, 1, T, 8, 2, false, P>, score32b_kernel<R, O, false, 1, T, 16, 2, false, P>, \
+      score32b_kernel<R, O, false, 1, T, 32, 2, false, P>, score32b_kernel<R, O, false, 1, T, 16, 2, true, P> }
+
 const Variant kVariants[] = {
-    {"bcast-mfma16/ring3/occ4", SD_BCAST(3, 4, true, 4), 1},
     {"bcast-mfma16/ring2/occ4", SD_BCAST(2, 4, true, 4), 1},
     {"bcast/ring4/occ4", SD_BCAST(4, 4, false, 4), 1},
-    {"bcast/ring2/occ5", SD_BCAST(2, 5, false, 4), 1},
-    {"bcast/ring4/occ5", SD_BCAST(4, 5, false, 4), 1},
     {"bcast2/ring4/occ4", SD_BCAST2(4, 4, 4), 1, 2},
-    {"readlane/unroll4/occ4", SD_SAME((score32r_kernel<4, 4>)), kWaves},
-    {"readlane/unroll4/occ3", SD_SAME((score32r_kernel<4, 3>)), kWaves},
-    {"mfma/unroll4/occ3", SD_SAME((score32m_kernel<4, 3>)), kWaves},
+    {"bcast2-queue/ring4/occ4", SD_BCAST2P(4, 4, 4, 2), 1, 2, 2},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 }  // namespace
@@ -1422,22 +1124,49 @@ const char *score32_variant_name(int v) {
     return (v >= 0 && v < kNumVariants) ? kVariants[v].name : "?";
 }
 
+__global__ void zero_counter_kernel(unsigned *c) {
+    if (threadIdx.x == 0) *c = 0u;
+}
+
+// Resident one-wave workgroups of a persistent kernel: occupancy x CUs.
+static int resident_grid(void (*kernel)(ScoreArgs), int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)kernel, threads, 0) !=
+            hipSuccess)
+        return 0;
+    return cus * (per > 1 ? per : 1);
+}
+
 hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     if (a.n_work <= 0) return hipSuccess;
     if (variant < 0 || variant >= kNumVariants) return hipErrorInvalidValue;
+    if (kVariants[variant].persist == 2 && !a.counter) return hipErrorInvalidValue;
     // A launch holds fewer than 2^32 threads (2^26 waves); longer work lists
     // are cut into chunks (the kernels address partials by glowworm and
     // block, so a chunk only needs the offset work pointer).
-    constexpr int kChunk = (1 << 26) - kWaves;
+    constexpr int kChunk = (1 << 26) - 64;
     const int wpb = kVariants[variant].waves;
     const int kc = mode_chunks(a.nmr);
     const int ki = modes_by_component(a.nmr) ? 3 : (kc == 8 ? 0 : (kc == 16 ? 1 : 2));
     void (*kernel)(ScoreArgs) = kVariants[variant].kernel[ki];
+    const int persist = kVariants[variant].persist;
+    const int resident = persist ? resident_grid(kernel, 64 * wpb) : 0;
+    if (persist && resident <= 0) return hipErrorInvalidConfiguration;
     for (int base = 0; base < a.n_work; base += kChunk) {
         ScoreArgs part = a;
         part.work = a.work + 2 * (size_t)base;
         part.n_work = min(kChunk, a.n_work - base);
-        const int grid = (part.n_work + wpb - 1) / wpb;
+        int grid = (part.n_work + wpb - 1) / wpb;
+        if (persist) {
+            grid = min(grid, resident);
+            if (persist == 2) {
+                hipLaunchKernelGGL(zero_counter_kernel, dim3(1), dim3(64), 0, s, a.counter);
+                const hipError_t e = hipGetLastError();
+                if (e != hipSuccess) return e;
+            }
+        }
         hipLaunchKernelGGL(kernel, dim3(grid), dim3(64 * wpb), 0, s,
                            part);
         const hipError_t e = hipGetLastError();

@@ -14,9 +14,10 @@
 //     ltype_b i32 [nl]             type b (the FP32 form's offset into a tablep row)
 //   receptor tiles (FP32 guard form): 64 atoms per tile, grouped by bisection
 //     tiles   f32x4 [ntiles]       tile centre (x,y,z) and radius
-//     rrel    f64 [3][ntiles*64]   coordinates - tile centre
-//     rmod32  f32 [nmr][3][ntiles*64]
-//     rtype_s i32 [ntiles*64]      type*3528 (a tablep row) in tile order
+//     rmod32  f32 [nmr][3][ntiles*64]      modes by slot (the GEMM probe)
+//     rpos32  f32x4 [ntiles*64]            FP32 coordinates (atom test)
+//     rrel4   f64 [ntiles*64][4]           FP64 coordinates, table row type*3528
+//     rmodA   f32 [ntiles*64][rmod_stride] modes by slot (displacement)
 //     rperm   i32 [ntiles*64]      original atom index (-1 = padding)
 //   ligand blocks (FP32 guard form): 64 atoms per block, grouped by bisection
 //     lperm   i32 [nlblocks*64]    original atom index (-1 = padding)
@@ -87,18 +88,16 @@ struct DevComplex {
     // receptor tiles
     int ntiles;
     const float4 *tiles;
-    const double *rrel;      // [3][ntiles*64] coordinates - tile centre (f64)
-    const float *rmod32;     // [nmr][3][ntiles*64] modes (f32)
-    // the same per slot, for gathers by slot: near-test coordinates (f32,
-    // absolute, w = 0), coordinates and type row (f64 absolute x, y, z and
-    // the row rtype_s as a double) and modes (f32, in the order
+    const float *rmod32;     // [nmr][3][ntiles*64] modes (f32; the GEMM probe's B)
+    // per slot, for gathers by slot: near-test coordinates (f32, absolute,
+    // w = 0), coordinates and table row (f64 absolute x, y, z and the row
+    // type * kPRow as a double) and modes (f32, in the order
     // modes_by_component(nmr) gives, padded to 4 * mode_chunks(nmr) =
     // rmod_stride floats)
     const float4 *rpos32;
     const double *rrel4;
     const float *rmodA;
     int rmod_stride;
-    const int *rtype_s;
     const int *rperm;
     float rmode_max[SDGSO_MAX_MODES];  // max over atoms of |M_m,i|
     // ligand blocks
@@ -139,6 +138,8 @@ struct ScoreArgs {
     // with candidate tiles, candidate tiles, near receptor atoms, queued
     // pairs, in-cutoff pairs), each the sum over its slots
     unsigned long long *stats;
+    // FP32 form, work-queue variants: the queue's counter (one word)
+    unsigned *counter;
 };
 
 struct SelectMoveArgs {
@@ -188,6 +189,9 @@ hipError_t launch_coords(const CoordsArgs &a, int count, hipStream_t s);
 hipError_t launch_rmsd(const double *bb, int nb, int G, int n_swarms, double *rmsd, hipStream_t s);
 hipError_t launch_gather_probe(const float *table, unsigned n, int per_thread, float *out,
                                int blocks, hipStream_t s);
+hipError_t launch_anm_gemm_probe(const DevComplex *cx, const double *pos, int P, int G,
+                                 const int *pairs, size_t first, size_t count, int chunks,
+                                 float *out, hipStream_t s);
 
 void set_error(const std::string &msg);
 

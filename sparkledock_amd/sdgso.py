@@ -75,6 +75,7 @@ SIGNATURES = [
     ("sdgso_run_set_variant", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_run_variants", ctypes.c_int, [_VP, _I]),
     ("sdgso_probe_gather", ctypes.c_int, [ctypes.c_int64, _D]),
+    ("sdgso_probe_anm_gemm", ctypes.c_int, [_VP, _D]),
     ("sdgso_run_final_coords", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _D, _D]),
     ("sdgso_run_cluster", ctypes.c_int,
      [_VP, ctypes.c_int, _I, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double, _I, _I]),
@@ -331,6 +332,13 @@ class Run:
         _check(lib().sdgso_run_cluster(self.h, int(complex_index), _p(rb, _I), len(rb), _p(lb, _I),
                                        len(lb), float(cutoff), _p(cl, _I), _p(rep, _I)))
         return cl, rep
+
+    def probe_anm_gemm(self):
+        """Time the receptor displacement as an FP32 matrix-core GEMM over the
+        tiles each swarm can reach (a measurement; state unchanged)."""
+        r = np.zeros(5)
+        _check(lib().sdgso_probe_anm_gemm(self.h, _p(r)))
+        return dict(ms=r[0], bytes_written=r[1], flops=r[2], displacements=r[3], swarm_tiles=r[4])
 
     def filter_stats(self):
         """Counts of one FP32 scoring pass over the current poses: waves

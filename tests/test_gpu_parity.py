@@ -523,7 +523,7 @@ def test_filter_stats_consistent_with_pairs():
         run.set_variant(v)
         st = run.filter_stats()
         # a two-block variant's waves take blocks (0, 1), (2, 3) and (4)
-        two = name.startswith("bcast2/")
+        two = name.startswith("bcast2")
         assert st["waves"] == 3 * 24 * (-(-n_blocks // 2) if two else n_blocks)
         assert 0 < st["waves_with_candidates"] <= st["waves"]
         assert st["near_atoms"] <= 64 * st["candidate_tiles"]
@@ -595,3 +595,20 @@ def test_tune_per_complex():
     assert np.allclose(b["score"], a["score"], rtol=1e-9, atol=1e-9)
     runs[sdgso.FP32_GUARD].set_variant(len(names) - 1)
     assert runs[sdgso.FP32_GUARD].variants() == [len(names) - 1] * 2
+
+
+def test_anm_gemm_probe_counts():
+    """The matrix-core GEMM probe covers, for every swarm, the tiles its
+    glowworms can reach (at least every tile that holds a pair), 64 atoms x
+    G glowworms per (swarm, tile), and leaves the run's state unchanged."""
+    c = GC.case(61, 900, 260, 10, 10, 3, 40, swarm_radius=3.0)
+    pr = GC.Pair(c)
+    run, _ = make_run(pr, c, 2, form=sdgso.FP32_GUARD)
+    before = run.read()
+    res = run.probe_anm_gemm()
+    assert res["ms"] > 0 and res["swarm_tiles"] >= 3
+    assert res["displacements"] == res["swarm_tiles"] * 64 * 40
+    assert res["bytes_written"] == res["swarm_tiles"] * 2 * 32 * 192 * 4
+    after = run.read()
+    for key in before:
+        assert np.array_equal(before[key], after[key]), key
