@@ -299,7 +299,8 @@ def main():
             run.set_variant(args.variant)
         run.steps(1)
         run.sync()
-        log("profile step done: %d glowworms" % (sum(wl["n_swarms"]) * wl["G"]))
+        log("profile step done: %d glowworms, %d in-cutoff pairs in the scoring launch"
+            % (sum(wl["n_swarms"]) * wl["G"], int(run.read()["pairs"].sum())))
         # the random-gather ceiling kernel, for counters to compare against
         log("gather probe: %.3g gathers/s" % sdgso.probe_gather())
         return

@@ -1146,7 +1146,9 @@ hipError_t launch_score32(const ScoreArgs &a, int variant, hipStream_t s) {
     // A launch holds fewer than 2^32 threads (2^26 waves); longer work lists
     // are cut into chunks (the kernels address partials by glowworm and
     // block, so a chunk only needs the offset work pointer).
-    constexpr int kChunk = (1 << 26) - 64;
+    // a multiple of every variant's waves per workgroup; tests/test_gpu_configs.py
+    // checks the glowworms on both sides of this cut
+    constexpr int kChunk = (1 << 26) - 4;
     const int wpb = kVariants[variant].waves;
     const int kc = mode_chunks(a.nmr);
     const int ki = modes_by_component(a.nmr) ? 3 : (kc == 8 ? 0 : (kc == 16 ? 1 : 2));
