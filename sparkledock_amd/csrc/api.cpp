@@ -314,7 +314,7 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
             if (s < a1) {
                 const int i = order[s];
                 perm[s] = i;
-                ts[s] = type[i] * kPRow;
+                ts[s] = 4 * type[i] * kPRow;  // byte offset of the type's tablep row
                 for (int k = 0; k < 3; k++) rrel[(size_t)k * slots + s] = xyz[3 * (size_t)i + k] - (double)cf[k];
                 for (int m = 0; m < nm; m++)
                     for (int k = 0; k < 3; k++) {
@@ -517,7 +517,9 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
                     tabp[(size_t)ta * kPRow + bin * SDGSO_NTYPES + tb] =
                         tab[(size_t)ta * SDGSO_NTYPES * SDGSO_NBINS + tb * SDGSO_NBINS + bin];
         c->tablep.upload(tabp);
-        c->ltype_b.upload(lig_type, n_lig);
+        std::vector<int> lb4(n_lig);  // byte offset of each ligand type in a tablep row
+        for (int i = 0; i < n_lig; i++) lb4[i] = 4 * lig_type[i];
+        c->ltype_b.upload(lb4);
         DevComplex &d = c->dev;
         d.nr = n_rec;
         d.nl = n_lig;

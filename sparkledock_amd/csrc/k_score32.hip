@@ -26,8 +26,8 @@
 //
 // Evaluate (all 64 lanes busy): exact d2 from the fixed-point coordinates
 // (integer differences are exact and convert to FP32 exactly below 16 A), the
-// bin, and the table value; several gathers are in flight per lane before the
-// compensated FP32 adds (TwoSum per term; sum + error term kept to the end).  The FP32 d2 differs from the FP64 reference
+// bin, and the table value; several gathers are in flight per lane before
+// they are added into the lane's FP64 sum.  The FP32 d2 differs from the FP64 reference
 // d2 by at most 9.04e-5 + 3.1e-6 (k_r S_r + k_l S_l) A^2 (S the pose's
 // receptor and ligand displacement bounds, k the FP32 roundings per mode
 // term); each wave widens the run's band (default 1e-4 A^2) to that bound.  A
@@ -38,7 +38,7 @@
 // floor(2 sqrt(d2)) through dist_to_bins (raw v_sqrt_f32, 1 ulp, cannot cross
 // an integer there).
 //
-// Lanes (sum + error term) are combined in FP64 and written as one partial per
+// Lane sums are combined in FP64 and written as one partial per
 // (glowworm, block); finish32 adds a glowworm's partials in block order in
 // FP64, so the energy keeps its relative accuracy when the terms cancel.
 #include <string>
@@ -63,21 +63,10 @@ constexpr int kFar = 1 << 29;                 // far-away fixed-point coordinate
 // The table gathers read tablep ([a][bin][b]): row a*kPRow from the receptor
 // atom, b from the ligand atom, then bin*168.
 
-// Compensated FP32 sum (Neumaier's form of Kahan's method, with Knuth's
-// branch-free TwoSum for each term's rounding error): sum + comp is the exact
-// sum of the terms up to O(n^2 u^2) sum |T|, so a sum whose terms cancel keeps
-// its relative accuracy.  A term not taken is added as 0, which is exact.
-struct CompSum {
-    float sum = 0.f, comp = 0.f;
-    __device__ __forceinline__ void add(float v, bool take) {
-        v = take ? v : 0.f;
-        const float t = sum + v;
-        const float vb = t - sum;
-        const float err = (sum - (t - vb)) + (v - vb);
-        sum = t;
-        comp += err;
-    }
-};
+// Each lane adds its table values (FP32, so exact in FP64) into one FP64
+// sum: a convert and an FP64 add per term, which issue at the FP32 rate on
+// this GPU.  A lane's n terms err by at most n 2^-53 sum |T|, so a sum whose
+// terms cancel keeps its relative accuracy.
 
 // x/y/z += M[m][.][i] * e_m for m in mode order, one mode at a time (the
 // reference order of prepare_gso_tasks.cc:465-504, as modes_add64).
@@ -95,14 +84,14 @@ __device__ __forceinline__ void modes_add64_lean(const double *modes, int n, int
 }
 
 // FP64 decision for a pair inside the guard band of a boundary, rebuilt from
-// the pose in the reference's order: returns (table value, 1) for a pair
-// inside (0, 225], else (0, 0).  Called from the evaluation loop of the FP32
-// kernels for about 2e-5 of the pairs, so it is kept out of line and lean:
-// its mode sums run one mode at a time, which leaves most of the caller's
-// registers untouched across the call.
-__device__ __noinline__ float2 guard_pair64(const DevComplex &c, const double *pose,
-                                            int lig_atom, int rec_atom) {
-    if (lig_atom < 0 || rec_atom < 0) return make_float2(0.f, 0.f);
+// the pose in the reference's order: returns (FP64 table value, 1) for a
+// pair inside (0, 225], else (0, 0).  Called from the evaluation
+// loop of the FP32 kernels for about 4e-5 of the pairs, so it is kept out of
+// line and lean: its mode sums run one mode at a time, which leaves most of
+// the caller's registers untouched across the call.
+__device__ __noinline__ double2 guard_pair64(const DevComplex &c, const double *pose,
+                                             int lig_atom, int rec_atom) {
+    if (lig_atom < 0 || rec_atom < 0) return make_double2(0.0, 0.0);
     double Rx = gld(c.r0, rec_atom), Ry = gld(c.r0, c.nr + rec_atom),
            Rz = gld(c.r0, 2 * c.nr + rec_atom);
     modes_add64_lean(c.rmodes, c.nr, rec_atom, pose + 7, c.nmr, Rx, Ry, Rz);
@@ -118,7 +107,7 @@ __device__ __noinline__ float2 guard_pair64(const DevComplex &c, const double *p
     double v64 = 0.0;
     const bool hit = pair_term64(Rx, Ry, Rz, Lx, Ly, Lz, gld(c.rtype, rec_atom),
                                  gld(c.ltype, lig_atom), c.table, v64);
-    return make_float2(hit ? (float)v64 : 0.f, hit ? 1.f : 0.f);
+    return make_double2(hit ? v64 : 0.0, hit ? 1.0 : 0.0);
 }
 
 __device__ __forceinline__ int qround(double v) { return (int)rint(v * (double)kQ); }
@@ -127,6 +116,24 @@ __device__ __forceinline__ void wave_sync_lds() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// bin_of_d in one v_med3_i32: max(0, min(d - 2, (d + 11) >> 1)) equals
+// med3(d - 2, 0, (d + 11) >> 1) for d >= -1, since (d + 11) >> 1 > 0 there:
+// d - 2 < 0 gives 0 both ways, and otherwise the median of 0 <= d - 2 and
+// the positive (d + 11) >> 1 is their minimum.
+__device__ __forceinline__ int bin_med3(int d) {
+    int r;
+    asm("v_med3_i32 %0, %1, 0, %2" : "=v"(r) : "v"(d - 2), "v"((d + 11) >> 1));
+    return r;
+}
+
+// a in the lanes whose bit of mask is set, else b: one v_cndmask_b32 on the
+// mask as a scalar register pair (a plain ?: may become an exec-mask branch).
+__device__ __forceinline__ unsigned select_lanes(unsigned long long mask, unsigned a, unsigned b) {
+    unsigned r;
+    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
+    return r;
 }
 
 // Classification of one exact fixed-point d2: bin of a pair outside every
@@ -144,7 +151,7 @@ __device__ __forceinline__ PairClass classify(float d2q, float bandq) {
     // boundaries are k = 0..30; a pair beyond 225 + band is simply out
     p.near = s < 30.5f && fabsf(diff) < bandq;
     p.take = !p.near && d2q <= k225q;
-    p.bin = bin_of_d((int)s - 1);
+    p.bin = bin_med3((int)s - 1);
     return p;
 }
 
@@ -274,11 +281,11 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
 // The wave's energy and pair count go to block x.b's partial; a second block
 // b2 (>= 0) of a two-block wave gets zeros.
 __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx &x,
-                                               const CompSum &acc, int npairs,
+                                               double acc, int npairs,
                                                unsigned int nguard, long long n_cand,
                                                long long n_near, long long n_queued,
                                                int b2 = -1) {
-    double e = (double)acc.sum + (double)acc.comp;
+    double e = acc;
     long long np = npairs;
     for (int off = 32; off > 0; off >>= 1) {
         e += __shfl_xor(e, off, 64);
@@ -433,7 +440,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     __shared__ unsigned short s_rs[kWpb][2 * 64];  // receptor slots of the packed atoms
     __shared__ PdT s_pd[kWpb][kPend];        // near slots waiting for displacement
     __shared__ int4 s_lg[kWpb][64 * kLB];    // the blocks' ligand atoms: fixed point + type*20
-    __shared__ int s_q[kWpb][kQueue];        // queued pairs (LDS offsets of both atoms)
+    __shared__ int s_q[kWpb][kQueue + 64];   // queued pairs (LDS offsets of both atoms), 64 dummy slots
     __shared__ float s_ef[kWpb][SDGSO_MAX_MODES];
     __shared__ unsigned short s_ct[kWpb][kCandB];  // candidate tiles, scanned in chunks
     __shared__ float4 s_box[kWpb][5];        // the box test's parameters (read once per batch)
@@ -458,6 +465,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     PdT *pd = s_pd[w];
     int4 *lg = s_lg[w];
     int *qq = s_q[w];
+    const unsigned q_dummy = (unsigned)(kQueue + (int)(threadIdx.x & 63)) << 2;  // this lane's dummy slot (bytes)
     float *ef = s_ef[w];
     unsigned short *ct = s_ct[w];
     WaveCtx x;
@@ -577,7 +585,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         }
     }
 
-    CompSum acc;
+    double acc = 0.0;
     int npairs = 0;          // in-cutoff pairs evaluated by this lane
     unsigned int nguard = 0;
     int qn = 0;
@@ -597,19 +605,22 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         const int4 r = *(const int4 *)(rt_base + ((unsigned)tag >> 16));
         const int4 l = *(const int4 *)(lg_base + (tag & 0xffff));
         const PairClass pc = classify(d2_fixed(l, r), bandq);
+        // r.w and l.w are byte offsets of the receptor type's row and the
+        // ligand type's column; a bin is 4 * 168 bytes
         const unsigned off =
-            pc.take ? 4u * (unsigned)(r.w + l.w + SDGSO_NTYPES * pc.bin) : kNoLoad;
-        acc.add(ring[j], true);
+            pc.take ? __umul24((unsigned)pc.bin, 4u * SDGSO_NTYPES) + (unsigned)(r.w + l.w)
+                    : kNoLoad;
+        acc += (double)ring[j];
         ring[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tbl, off, 0, 0));
         npairs += pc.take ? 1 : 0;
         if (pc.near) {
             // ligand slot li of block x.b (li >= 64: slot li - 64 of block x.b + 1)
             const int li = (int)((unsigned)(tag & 0xffff) >> 4) - w * 64 * kLB;
             const int ki = (int)((unsigned)tag >> 20) - w * kRtRows;
-            const float2 g2 = guard_pair64(c, x.pose, gld(c.lperm, x.b * kBlock + li),
-                                           gld(c.rperm, (int)rs[ki]));
-            acc.add(g2.x, g2.y != 0.f);
-            npairs += g2.y != 0.f ? 1 : 0;
+            const double2 g2 = guard_pair64(c, x.pose, gld(c.lperm, x.b * kBlock + li),
+                                            gld(c.rperm, (int)rs[ki]));
+            acc += g2.x;
+            npairs += g2.y != 0.0 ? 1 : 0;
             nguard++;
         }
     };
@@ -867,11 +878,13 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                         const bool hit = d[v] <= thr16;
                         const unsigned long long m = __ballot(hit);
                         if (m) {
-                            if (hit)
-                                qq[qn + __builtin_amdgcn_mbcnt_hi(
-                                            (unsigned)(m >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
+                            const unsigned qb = __builtin_amdgcn_readfirstlane((unsigned)qn << 2);
+                            if (hit) {
+                                const unsigned rank = __builtin_amdgcn_mbcnt_hi(
+                                    (unsigned)(m >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u));
+                                *(int *)((char *)qq + ((rank << 2) + qb)) =
                                     (int)(tagb + ((16u * (unsigned)(8 * (v >> 2) + (v & 3))) << 16));
+                            }
                             qn += __popcll(m);
                         }
                     }
@@ -884,7 +897,9 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             // holds a far-away atom that never passes)
             auto pretest = [&](int i_begin, int i_end, float ox, float oy, float oz, float ot,
                                unsigned olo) {
-                for (int i0 = i_begin; i0 < i_end; i0 += 4) {
+                // queue word of atom i0 + j: tr + (j << 20), tr kept per round
+                unsigned tr = ((hib + 16u * (unsigned)i_begin) << 16) | olo;
+                for (int i0 = i_begin; i0 < i_end; i0 += 4, tr += 4u << 20) {
                     float4 r[4];
 #pragma unroll
                     for (int j = 0; j < 4; j++) r[j] = rf[i0 + j];
@@ -897,14 +912,18 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                         in[j] = wv2 <= ot;
                         m[j] = __ballot(in[j]);
                     }
+                    // for an atom with passing pairs every lane stores: a
+                    // passing pair at the queue's end (qb, bytes) plus its
+                    // rank, the others into the wave's dummy slots past the
+                    // queue, so there is no exec mask change per atom
 #pragma unroll
                     for (int j = 0; j < 4; j++) {
                         if (m[j]) {
-                            if (in[j])
-                                qq[qn + __builtin_amdgcn_mbcnt_hi(
-                                            (unsigned)(m[j] >> 32),
-                                            __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
-                                    (int)(((hib + 16u * (unsigned)(i0 + j)) << 16) | olo);
+                            const unsigned qb = __builtin_amdgcn_readfirstlane((unsigned)qn << 2);
+                            const unsigned rank = __builtin_amdgcn_mbcnt_hi(
+                                (unsigned)(m[j] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u));
+                            const unsigned addr = select_lanes(m[j], (rank << 2) + qb, q_dummy);
+                            *(int *)((char *)qq + addr) = (int)(tr + ((unsigned)j << 20));
                             qn += __popcll(m[j]);
                         }
                     }
@@ -931,11 +950,14 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     if (m[j]) {
-                        if (in[j])
-                            qq[qn + __builtin_amdgcn_mbcnt_hi(
-                                        (unsigned)(m[j] >> 32),
-                                        __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u))] =
-                                (int)(((hib + 16u * (unsigned)(i0 + j)) << 16) | lane_lo);
+                        const unsigned hi = __builtin_amdgcn_readfirstlane(
+                            (hib + 16u * (unsigned)(i0 + j)) << 16);
+                        const unsigned qb = __builtin_amdgcn_readfirstlane((unsigned)qn << 2);
+                        if (in[j]) {
+                            const unsigned rank = __builtin_amdgcn_mbcnt_hi(
+                                (unsigned)(m[j] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u));
+                            *(int *)((char *)qq + ((rank << 2) + qb)) = (int)(hi | lane_lo);
+                        }
                         qn += __popcll(m[j]);
                     }
                 }
@@ -1024,7 +1046,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     }
     flush_all();
 #pragma unroll
-    for (int j = 0; j < kUnroll; j++) acc.add(ring[j], true);
+    for (int j = 0; j < kUnroll; j++) acc += (double)ring[j];
     write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued, b2);
     };
     if constexpr (kPersist == 0) {

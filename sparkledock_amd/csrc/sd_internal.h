@@ -11,12 +11,12 @@
 //     table64 f64 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
 //     table   f32 [168*168*20+20]  the same rounded to FP32 (FP32 guard form)
 //     tablep  f32 [168*3528+64]    the FP32 values as [a][bin 0..20][b] (FP32 form)
-//     ltype_b i32 [nl]             type b (the FP32 form's offset into a tablep row)
+//     ltype_b i32 [nl]             4 * type b (byte offset into a tablep row)
 //   receptor tiles (FP32 guard form): 64 atoms per tile, grouped by bisection
 //     tiles   f32x4 [ntiles]       tile centre (x,y,z) and radius
 //     rmod32  f32 [nmr][3][ntiles*64]      modes by slot (the GEMM probe)
 //     rpos32  f32x4 [ntiles*64]            FP32 coordinates (atom test)
-//     rrel4   f64 [ntiles*64][4]           FP64 coordinates, table row type*3528
+//     rrel4   f64 [ntiles*64][4]           FP64 coordinates, tablep row's byte offset 4*type*3528
 //     rmodA   f32 [ntiles*64][rmod_stride] modes by slot (displacement)
 //     rperm   i32 [ntiles*64]      original atom index (-1 = padding)
 //   ligand blocks (FP32 guard form): 64 atoms per block, grouped by bisection
@@ -83,7 +83,7 @@ struct DevComplex {
     const float *table;      // FP32 copy (FP32 form, its guard-band pairs)
     const double *table64;   // the caller's FP64 values (FP64 form)
     const float *tablep;     // FP32 copy in the [a][bin][b] order (FP32 form)
-    const int *ltype_b;      // [nl] ligand type b (offset into tablep rows)
+    const int *ltype_b;      // [nl] 4 * ligand type b (byte offset into tablep rows)
     double lref[3];
     // receptor tiles
     int ntiles;
