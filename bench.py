@@ -453,7 +453,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 (FP64 guard-band recompute, compensated sums)",
+        "dtype": "f32 (FP64 guard-band recompute, FP64 lane sums)",
         "data": "synthetic, seeded (random-walk globules, N(0,1)*0.5 table)",
         "config": {"workload": args.workload, "complexes": len(wl["complexes"]),
                    "swarms": int(sum(wl["n_swarms"])), "glowworms_per_swarm": wl["G"],

@@ -49,7 +49,7 @@ enum {
     SDGSO_FP64 = 0,       /* plain FP64 brute force over all Nr*Nl pairs */
     SDGSO_FP32_GUARD = 1  /* FP32 distances on receptor tiles, FP64 recompute of
                              pairs within the guard band of a bin boundary,
-                             compensated FP32 energy sums */
+                             FP32 table values summed in FP64 */
 };
 
 /* GSO constants (prepare_gso_tasks.cc:386-400; steps :783, 791-794). */

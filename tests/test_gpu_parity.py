@@ -3,8 +3,8 @@
 Bars: FP64 form bit-exact (scores, pair counts, reference points); FP32 guard
 form with identical DFIRE bins (per-bin pair counts equal) and energy within
 1e-9 of the oracle relative to |E| plus 1e-12 of the sum of |T| over in-cutoff
-pairs (compensated FP32 sums keep the relative accuracy of a sum whose terms
-cancel; the floor covers E near 0); GSO
+pairs (FP64 lane sums of the FP32 table values keep the relative accuracy of
+a sum whose terms cancel; the floor covers E near 0); GSO
 choices (neighbour counts, selections, moved flags) bit-exact from identical
 states; poses equal up to the slerp's acos/sin (GPU libm vs glibc): 1e-14.
 """
@@ -126,7 +126,7 @@ def test_ligand_block_counts(nl):
 def test_energy_with_terms_of_both_signs_over_eight_decades():
     """Table values of both signs spread over eight orders of magnitude: the
     large terms swamp the small ones in a plain or Kahan FP32 sum (errors of
-    about 2^-24 of the running sum).  The compensated sums still hold every
+    about 2^-24 of the running sum).  The FP64 lane sums still hold every
     variant's energy to the oracle's FP64 sum within 1e-9 |E| + 1e-12 sum |T|."""
     c = GC.case(31, 700, 300, 6, 4, 3, 16)
     rng = np.random.default_rng(31)
