@@ -281,11 +281,9 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
     std::vector<double> rrel(3 * slots, 0.0);
     std::vector<float> m32((size_t)nm * 3 * slots, 0.0f);
     std::vector<int> ts(slots, 0), perm(slots, -1);
-    // modes of a slot: 3*nm floats in mode_chunks(nm) float4, component-major
-    // (each coordinate in q4 floats) or mode-major, as modes_by_component says
+    // modes of a slot: 3*nm floats in mode_chunks(nm) float4, in the order
+    // mode_slot_index gives
     const int stride = 4 * mode_chunks(nm);
-    const bool bycomp = modes_by_component(nm);
-    const int q4 = 4 * ((nm + 3) / 4);
     std::vector<float4> p32(slots);
     std::vector<double> r4(4 * slots, 0.0);
     std::vector<float> mA((size_t)stride * slots, 0.0f);
@@ -320,7 +318,7 @@ static void build_tiles(sdgso_complex c, const double *xyz, const int *type, con
                     for (int k = 0; k < 3; k++) {
                         const float mv = (float)msoa[((size_t)m * 3 + k) * n + i];
                         m32[((size_t)m * 3 + k) * slots + s] = mv;
-                        mA[(size_t)s * stride + (bycomp ? k * q4 + m : 3 * m + k)] = mv;
+                        mA[(size_t)s * stride + mode_slot_index(nm, m, k)] = mv;
                     }
             } else {
                 // padding slot: far away, type 0, zero modes

@@ -742,6 +742,60 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             }
             wave_sync_lds();
             if (lane < n) D = rf[lane];
+        } else if (kC == 8 && nmr > 0) {
+            // packed by coordinate (up to 10 modes): coordinate k holds floats
+            // [k cs, (k + 1) cs), cs = max(nmr, 4), so the float4 of lane j
+            // holds modes of its first coordinate ka and maybe of ka + 1.  The
+            // lane weighs its floats into a (coordinate ka) and b (ka + 1); b
+            // moves to lane j + 1, whose first coordinate is ka + 1 (a
+            // coordinate spans at least four floats); then a scan over the
+            // lanes of one first coordinate (at most three: shifts 1 and 2)
+            // leaves each coordinate's sum in its last lane
+            const int j = lane & 7;
+            const int cs = nmr > 4 ? nmr : 4;
+            const int ka = (4 * j) / cs;
+            float wa[4], wb[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int f = 4 * j + t, k = f / cs, m = f - k * cs;
+                const float e = k < 3 && m < nmr ? ef[min(m, SDGSO_MAX_MODES - 1)] : 0.f;
+                wa[t] = k == ka ? e : 0.f;
+                wb[t] = k == ka + 1 ? e : 0.f;
+            }
+            const float gb = j >= 1 && (4 * j - 1) / cs == ka && (4 * j - 4) / cs != ka ? 1.f : 0.f;
+            const float g1 = j >= 1 && (4 * j - 4) / cs == ka ? 1.f : 0.f;
+            const float g2 = j >= 2 && (4 * j - 8) / cs == ka ? 1.f : 0.f;
+            const bool last = ka < 3 && (j == 7 || (4 * j + 4) / cs != ka);
+            constexpr int kApr = 8;  // atoms per round
+            const int nround = (n + kApr - 1) / kApr;
+            for (int k0 = 0; k0 < nround; k0 += kDispRounds) {
+                float4 qv[kDispRounds];
+                int av[kDispRounds];
+#pragma unroll
+                for (int r = 0; r < kDispRounds; r++) {
+                    av[r] = (k0 + r) * kApr + lane / 8;
+                    qv[r] = (k0 + r < nround && av[r] < n)
+                                ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * 8 + j)
+                                : make_float4(0.f, 0.f, 0.f, 0.f);
+                }
+#pragma unroll
+                for (int r = 0; r < kDispRounds; r++) {
+                    if (k0 + r >= nround) break;
+                    float va = qv[r].x * wa[0];
+                    va = __builtin_fmaf(qv[r].y, wa[1], va);
+                    va = __builtin_fmaf(qv[r].z, wa[2], va);
+                    va = __builtin_fmaf(qv[r].w, wa[3], va);
+                    float vb = qv[r].y * wb[1];
+                    vb = __builtin_fmaf(qv[r].z, wb[2], vb);
+                    vb = __builtin_fmaf(qv[r].w, wb[3], vb);
+                    float v = __builtin_fmaf(dpp_f32<0x111>(vb), gb, va);  // row_shr:1
+                    v = __builtin_fmaf(dpp_f32<0x111>(v), g1, v);
+                    v = __builtin_fmaf(dpp_f32<0x112>(v), g2, v);          // row_shr:2
+                    if (last && av[r] < n) ((float *)&rf[av[r]])[ka] = v;
+                }
+            }
+            wave_sync_lds();
+            if (lane < n) D = rf[lane];
         } else if (nmr > 0) {
             const int j = lane & (kC - 1);
             float wx[4], wy[4], wz[4];

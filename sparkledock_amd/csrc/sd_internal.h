@@ -64,12 +64,20 @@ inline __host__ __device__ int mode_chunks(int nm) {
     return lines <= 1 ? 8 : (lines <= 2 ? 16 : 32);
 }
 
-// Order of a slot's FP32 modes in rmodA.  For 11 to 20 modes (16 float4 per
-// slot) component-major, [x of modes 0..nm-1 | y | z] with each coordinate
-// in ceil(nm / 4) whole float4 (zeros past nm): every float4 then holds modes
-// of one coordinate, and the FP32 kernels take a variant for that order.
-// Otherwise mode-major: x, y, z of mode 0, then of mode 1, and so on.
+// Order of a slot's FP32 modes in rmodA.
+//  - Up to 10 modes (8 float4 per slot): by coordinate, packed, [x of modes
+//    0..nm-1 | y | z] with each coordinate in max(nm, 4) consecutive floats,
+//    so a float4 holds modes of at most two coordinates.
+//  - 11 to 20 modes (16 float4): component-major, each coordinate in
+//    ceil(nm / 4) whole float4 (zeros past nm), so every float4 holds modes of
+//    one coordinate; the FP32 kernels take a variant for that order.
+//  - More: mode-major, x, y, z of mode 0, then of mode 1, and so on.
 inline __host__ __device__ bool modes_by_component(int nm) { return nm >= 11 && nm <= 20; }
+inline __host__ __device__ int mode_slot_index(int nm, int m, int k) {
+    if (nm <= 10) return k * (nm > 4 ? nm : 4) + m;
+    if (nm <= 20) return k * 4 * ((nm + 3) / 4) + m;
+    return 3 * m + k;
+}
 
 // Per-complex device descriptor (plain data; lives in a device array).
 struct DevComplex {

@@ -63,13 +63,15 @@ def test_score_fp64_bit_exact(seed, nmr, nml):
 
 @pytest.mark.parametrize("seed,nmr,nml", [(4, 10, 10), (5, 0, 0), (6, 20, 20), (9, 24, 6), (10, 32, 1),
                                           (11, 1, 0), (12, 8, 3), (13, 11, 2), (14, 16, 5),
-                                          (15, 21, 0), (16, 9, 9)])
+                                          (15, 21, 0), (16, 9, 9), (17, 2, 1), (18, 3, 4),
+                                          (19, 5, 2), (22, 6, 6), (23, 7, 7)])
 def test_score_fp32_guard(seed, nmr, nml):
     """Receptor mode counts cover every mode-line width of the broadcast
     variant's cooperative displacement, 8 float4 (up to 10 modes), 16 (up to
-    21) and 32 (up to 32), and none, and both orders of a slot's modes:
-    component-major with 3, 4 and 5 float4 per coordinate (11, 16, 20
-    modes) and mode-major (1, 8, 9, 10, 21, 24, 32)."""
+    21) and 32 (up to 32), and none, and the three orders of a slot's modes:
+    packed by coordinate for 1 to 10 modes (a float4 with modes of two
+    coordinates for 3, 5, 6, 7, 9 and 10), component-major with 3, 4 and 5
+    float4 per coordinate (11, 16, 20 modes) and mode-major (21, 24, 32)."""
     c = GC.case(seed, 700, 300, nmr, nml, 4, 16)
     pr = GC.Pair(c)
     poses = poses_of(c)
