@@ -249,8 +249,8 @@ def main():
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--anm-probe", action="store_true",
-                    help="also time the receptor displacement as an FP32 matrix-core GEMM "
+    ap.add_argument("--no-anm-probe", action="store_true",
+                    help="skip timing the receptor displacement as an FP32 matrix-core GEMM "
                          "(sdgso_probe_anm_gemm) on the timed run's poses")
     ap.add_argument("--no-c0", action="store_true",
                     help="skip the full c0 runs (CPU oracle and both GPU forms)")
@@ -352,11 +352,22 @@ def main():
     dte = timed_steps(run, cmp_steps, graph=False)
     filt = run.filter_stats()
     anm = None
-    if args.anm_probe:
-        anm = run.probe_anm_gemm()
+    if not args.no_anm_probe and max(cx.nmr for cx in wl["complexes"]) > 0:
+        anm = {k: float(v) for k, v in run.probe_anm_gemm().items()}
         anm["fused_displacements"] = filt["near_atoms"]
         anm["gemm_TFLOPs"] = anm["flops"] / (anm["ms"] * 1e-3) / 1e12
+        anm["peak_TFLOPs"] = FP32_PEAK_TFLOPS
+        anm["frac"] = anm["gemm_TFLOPs"] / FP32_PEAK_TFLOPS
         anm["write_GBps"] = anm["bytes_written"] / (anm["ms"] * 1e-3) / 1e9
+        # the whole (poses x modes) . (modes x 3N) products of one step, as the
+        # reference materialises them (receptor and ligand)
+        anm["full_gemm_flops_per_step"] = float(sum(
+            2.0 * S * wl["G"] * (cx.nmr * 3 * cx.nr + cx.nml * 3 * cx.nl)
+            for cx, S in zip(wl["complexes"], wl["n_swarms"])))
+        anm["form"] = ("receptor displacement of the tiles each swarm reaches as "
+                       "v_mfma_f32_32x32x2f32 products, written as FP32 coordinates; the "
+                       "scoring kernel instead displaces the fused_displacements atoms that "
+                       "pass its atom test")
         log("anm gemm probe:", anm)
     log("fp32 guard: %.2f ms/step (profiled graph replays: scoring %.2f ms + rest %.2f ms); "
         "one-graph replays %.2f ms/step, eager %.2f ms/step; pairs/launch %.3e, guard "
