@@ -58,7 +58,6 @@ constexpr float kInvQ2x2 = 1.0f / 524288.0f;  // sqrt(d2q) * 2^-19 = 2*sqrt(d2)
 constexpr float kQ2over4 = 274877906944.0f;   // 2^38 = 2^40 / 4
 constexpr float kQ2 = 1099511627776.0f;       // 2^40
 constexpr float k225q = 225.0f * kQ2;         // the cutoff in 2^-40 A^2 units (exact)
-constexpr float kFilterMargin = 0.05f;        // A^2, over the MFMA filter's rounding
 constexpr int kFar = 1 << 29;                 // far-away fixed-point coordinate (512 A)
 // The table gathers read tablep ([a][bin][b]): row a*kPRow from the receptor
 // atom, b from the ligand atom, then bin*168.
