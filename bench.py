@@ -499,6 +499,10 @@ def main():
         "ranking": rank_cmp,
         "guard_recomputes_per_step": guard["recomputed"] / max(1, args.steps),
         "pairs_per_eval": pairs_per_launch / n_glow,
+        "evaluations": {"policy": "one per glowworm per step (swarms x glowworms x steps), as the "
+                                  "reference GPU path scores; the reference CPU path re-scores only "
+                                  "the glowworms that moved",
+                        "moved_fraction_last_step": float(np.mean(st["moved"]))},
         "filter": filt,
         "per_complex": per_complex,
         "anm_gemm_probe": anm,
