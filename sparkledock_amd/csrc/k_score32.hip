@@ -127,14 +127,6 @@ __device__ __forceinline__ int bin_med3(int d) {
     return r;
 }
 
-// a in the lanes whose bit of mask is set, else b: one v_cndmask_b32 on the
-// mask as a scalar register pair (a plain ?: may become an exec-mask branch).
-__device__ __forceinline__ unsigned select_lanes(unsigned long long mask, unsigned a, unsigned b) {
-    unsigned r;
-    asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(r) : "v"(b), "v"(a), "s"(mask));
-    return r;
-}
-
 // Classification of one exact fixed-point d2: bin of a pair outside every
 // guard band (take), or a pair inside a band (near: decided in FP64).
 struct PairClass {
@@ -975,7 +967,12 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                             const unsigned qb = __builtin_amdgcn_readfirstlane((unsigned)qn << 2);
                             const unsigned rank = __builtin_amdgcn_mbcnt_hi(
                                 (unsigned)(m[j] >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)m[j], 0u));
-                            const unsigned addr = select_lanes(m[j], (rank << 2) + qb, q_dummy);
+                            // the address is formed in every lane (the empty asm
+                            // keeps it from moving under an exec mask), then one
+                            // v_cndmask on the compare's lane mask picks it
+                            unsigned hit_addr = (rank << 2) + qb;
+                            asm volatile("" : "+v"(hit_addr));
+                            const unsigned addr = in[j] ? hit_addr : q_dummy;
                             *(int *)((char *)qq + addr) = (int)(tr + ((unsigned)j << 20));
                             qn += __popcll(m[j]);
                         }
