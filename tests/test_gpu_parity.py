@@ -89,6 +89,27 @@ def test_score_fp32_guard(seed, nmr, nml):
         assert_energy(e, e0, denom, v)
 
 
+@pytest.mark.parametrize("nmr", list(range(0, 33)))
+def test_score_fp32_every_receptor_mode_count(nmr):
+    """Every receptor mode count the kernels take (0 to 32, each with its
+    record width and order: packed by coordinate up to 10, component-major
+    11 to 20, mode-major above) and a ligand mode count that varies with it,
+    every variant against the oracle: pair counts equal, energies within the
+    FP32 bound."""
+    nml = (7 * nmr + 3) % 33
+    c = GC.case(100 + nmr, 500, 200, nmr, nml, 2, 8)
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, pairs0 = pr.oracle_scores(poses)
+    denom = GC.sum_abs_terms(pr, poses)
+    e0 = (4.7 - s0) / 0.0157
+    assert pairs0.sum() > 0
+    for v in VARIANTS:
+        s, pairs = score32(pr.gpu, poses, v)
+        assert np.array_equal(pairs, pairs0), (nmr, nml, v)
+        assert_energy((4.7 - s) / 0.0157, e0, denom, v)
+
+
 def per_bin_counts(pr, poses, form, variant=None):
     """Pair count of every bin through one-hot tables (exact integer sums);
     `variant` picks an FP32 kernel variant (default: the plugin call)."""
