@@ -173,10 +173,9 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
  * compiled variants (ligand-block size x filter form x gathers in flight per
  * lane x register/occupancy target x work distribution); a name starting
  * with "bcast2" is a variant whose wave takes a 128-atom ligand block (two
- * 64-atom blocks of one glowworm), the others take 64 atoms; "-stride" and
- * "-queue" name the persistent forms (a grid of resident waves taking the
- * work list by a fixed stride, or 8 items at a time from an atomic counter),
- * the others one work item per wave.  sdgso_run_tune scores the current poses of
+ * 64-atom blocks of one glowworm), the others take 64 atoms; "-queue" names
+ * the persistent form (a grid of resident waves taking 8 work items at a time
+ * from an atomic counter), the others take one work item per wave.  sdgso_run_tune scores the current poses of
  * each complex of the run with each variant `repeats` times (luciferin and
  * step counters untouched) and keeps, per complex, the fastest for later
  * steps: times_ms[n_complex][sdgso_tune_variants()] receives the
