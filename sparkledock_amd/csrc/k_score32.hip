@@ -590,11 +590,19 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
 
     // One round: the 64 queued pairs from entry e0, table load into ring slot j
     // (whose previous load is added first).
-    auto round = [&](auto jc, int e0) {
+    // A round's inputs: the queue word and both packed atoms.
+    struct RoundIn {
+        int tag;
+        int4 r, l;
+    };
+    auto fetch_atoms = [&](RoundIn &in) {
+        in.r = *(const int4 *)(rt_base + ((unsigned)in.tag >> 16));
+        in.l = *(const int4 *)(lg_base + (in.tag & 0xffff));
+    };
+    auto compute = [&](auto jc, const RoundIn &in) {
         constexpr int j = decltype(jc)::value;
-        const int tag = qq[e0 + lane];
-        const int4 r = *(const int4 *)(rt_base + ((unsigned)tag >> 16));
-        const int4 l = *(const int4 *)(lg_base + (tag & 0xffff));
+        const int tag = in.tag;
+        const int4 r = in.r, l = in.l;
         const PairClass pc = classify(d2_fixed(l, r), bandq);
         // r.w and l.w are byte offsets of the receptor type's row and the
         // ligand type's column; a bin is 4 * 168 bytes
@@ -615,11 +623,32 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             nguard++;
         }
     };
+    // One round: the 64 queued pairs from entry e0, table load into ring slot j
+    // (whose previous load is added first).
+    auto round = [&](auto jc, int e0) {
+        RoundIn in;
+        in.tag = qq[e0 + lane];
+        fetch_atoms(in);
+        compute(jc, in);
+    };
     // Whole groups of kUnroll rounds: every slot is reloaded in every group, so
     // a slot's load has the other kUnroll - 1 loads behind it when it is added.
+    // The LDS reads run ahead of the arithmetic: the queue word two rounds
+    // ahead and the atoms one round ahead.
     auto eval_groups = [&](int ng) {
-        for (int g = 0; g < ng; g++)
-            static_for<0, kUnroll>([&](auto jc) { round(jc, kGroup * g + 64 * decltype(jc)::value); });
+        for (int g = 0; g < ng; g++) {
+            const int e = kGroup * g;
+            RoundIn in[kUnroll];
+            in[0].tag = qq[e + lane];
+            if constexpr (kUnroll > 1) in[1].tag = qq[e + 64 + lane];
+            fetch_atoms(in[0]);
+            static_for<0, kUnroll>([&](auto jc) {
+                constexpr int j = decltype(jc)::value;
+                if constexpr (j + 2 < kUnroll) in[j + 2].tag = qq[e + 64 * (j + 2) + lane];
+                if constexpr (j + 1 < kUnroll) fetch_atoms(in[j + 1]);
+                compute(jc, in[j]);
+            });
+        }
     };
     // Rounds 0..nround-1, for the last flush of a batch or of the wave.
     auto eval_tail = [&](int nround) {
