@@ -310,7 +310,7 @@ def main():
     # replays, then replays of the one-graph step and eager steps to compare
     cmp_steps = max(1, min(args.steps, 5))
     wl = workloads.build(args.workload, swarm_scale=args.swarm_scale,
-                         steps=args.warmup + 2 + args.steps + 2 * cmp_steps)
+                         steps=args.warmup + 2 + args.steps + 2 * cmp_steps + 3)
     n_glow = sum(wl["n_swarms"]) * wl["G"]
     log("workload %s: %d complexes, %d glowworms, built in %.1f s" %
         (args.workload, len(wl["complexes"]), n_glow, time.perf_counter() - t0))
@@ -346,6 +346,8 @@ def main():
     pairs_per_launch = float(st["pairs"].sum())
     score_ms = prof["score_ms"] / max(1, prof["score_launches"])
     rest_ms = prof["move_ms"] / max(1, prof["move_launches"])
+    lgemm_ms = prof["lgemm_ms"] / max(1, prof["lgemm_launches"])
+    lgemm_on, lgemm_bytes = run.ligand_gemm()
     # the same step as one graph (no events) and eagerly, for comparison
     run.steps(1, graph=True)
     dt1 = timed_steps(run, cmp_steps, graph=True)
@@ -369,10 +371,45 @@ def main():
                        "scoring kernel instead displaces the fused_displacements atoms that "
                        "pass its atom test")
         log("anm gemm probe:", anm)
-    log("fp32 guard: %.2f ms/step (profiled graph replays: scoring %.2f ms + rest %.2f ms); "
-        "one-graph replays %.2f ms/step, eager %.2f ms/step; pairs/launch %.3e, guard "
-        "recomputes %d" % (1e3 * dt / args.steps, score_ms, rest_ms, 1e3 * dt1 / cmp_steps,
-                           1e3 * dte / cmp_steps, pairs_per_launch, guard["recomputed"]))
+    log("fp32 guard: %.2f ms/step (profiled graph replays: ligand GEMM %.2f ms + scoring %.2f "
+        "ms + rest %.2f ms); one-graph replays %.2f ms/step, eager %.2f ms/step; pairs/launch "
+        "%.3e, guard recomputes %d" % (1e3 * dt / args.steps, lgemm_ms, score_ms, rest_ms,
+                                       1e3 * dt1 / cmp_steps, 1e3 * dte / cmp_steps,
+                                       pairs_per_launch, guard["recomputed"]))
+    # the ligand displacement GEMM, (poses x modes) . (modes x 3 * 64 slots) per
+    # ligand block on the matrix cores with its FP32 output written to HBM: the
+    # tuner keeps it when a scoring pass with it is faster; when it did not,
+    # three more profiled replays with it on measure it (after the timed region)
+    lgemm = None
+    if max(cx.nml for cx in wl["complexes"]) > 0:
+        t_with, t_without = run.ligand_gemm_tune_ms()
+        lg = dict(ms=lgemm_ms, score_ms=score_ms, bytes=lgemm_bytes)
+        if not lgemm_on:
+            run.set_ligand_gemm(True)
+            run.profile(True)
+            run.steps(1, graph=True)
+            run.sync()
+            run.profile(True)
+            run.steps(2, graph=True)
+            p2 = run.profile_read()
+            lg = dict(ms=p2["lgemm_ms"] / max(1, p2["lgemm_launches"]),
+                      score_ms=p2["score_ms"] / max(1, p2["score_launches"]),
+                      bytes=run.ligand_gemm()[1])
+            run.profile(False)
+            run.set_ligand_gemm(False)
+        lg_flops = float(sum(2.0 * S * wl["G"] * cx.nml * 3 * 64 * -(-cx.nl // 64)
+                             for cx, S in zip(wl["complexes"], wl["n_swarms"])))
+        lgemm = {"on_in_timed_steps": lgemm_on,
+                 "tuner_ms_per_scoring_pass": {"with": t_with, "without": t_without},
+                 "ms": lg["ms"], "scoring_ms_with_it": lg["score_ms"],
+                 "bytes_written": lg["bytes"],
+                 "write_GBps": lg["bytes"] / (lg["ms"] * 1e-3) / 1e9,
+                 "write_frac_of_hbm_peak": lg["bytes"] / (lg["ms"] * 1e-3) / 8e12,
+                 "flops": lg_flops, "TFLOPs": lg_flops / (lg["ms"] * 1e-3) / 1e12,
+                 "frac_of_fp32_mfma_peak": lg_flops / (lg["ms"] * 1e-3) / 1e12 / FP32_PEAK_TFLOPS,
+                 "kernel": "ligand_disp_kernel (v_mfma_f32_32x32x2f32)",
+                 "bound": "hbm write of its output"}
+        log("ligand gemm:", lgemm)
 
     # ---- plain FP64 per-complex form on the same workload
     fp64 = None
@@ -455,9 +492,11 @@ def main():
         "ms_per_step": 1e3 * dt / args.steps,
         "gso_steps_per_s": args.steps / dt,
         "timing": {"timed": "%d replays of the captured step (capture, instantiation and a "
-                            "first replay before t0), each as the scoring graph and the graph "
-                            "of the rest of the step with HIP events between them"
+                            "first replay before t0), each as the graphs of the ligand "
+                            "displacement GEMM, the scoring launches and the rest of the step, "
+                            "with HIP events between them"
                             % args.steps,
+                   "ligand_gemm_ms": lgemm_ms if lgemm_on else None,
                    "scoring_ms": score_ms, "rest_ms": rest_ms,
                    "one_graph_ms_per_step": 1e3 * dt1 / cmp_steps,
                    "eager_ms_per_step": 1e3 * dte / cmp_steps, "compare_steps": cmp_steps},
@@ -505,6 +544,7 @@ def main():
                         "moved_fraction_last_step": float(np.mean(st["moved"]))},
         "filter": filt,
         "per_complex": per_complex,
+        "ligand_gemm": lgemm,
         "anm_gemm_probe": anm,
         "autotune": {"chosen_per_complex": chosen_names,
                      "ms_per_launch": ({k: sum(v) for k, v in tune_ms.items()} if tune_ms else None)},

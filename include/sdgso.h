@@ -132,7 +132,8 @@ int sdgso_run_set_stream(sdgso_run r, void *hip_stream);
 /* Advance n steps (asynchronous on the run's stream).  use_graph != 0 replays
  * the step as a captured hipGraph (captured at the first such call; while
  * profiling is enabled, as two graphs, the scoring launches and the rest of
- * the step, with the timing events between them). */
+ * the step, with the timing events between them, and a third in front of
+ * them when the ligand displacement GEMM is on). */
 int sdgso_run_steps(sdgso_run r, int n, int use_graph);
 
 /* Phases of one step, for single-phase tests: bit 0 score + luciferin,
@@ -159,13 +160,37 @@ int sdgso_run_write(sdgso_run r, const double *pos, const double *luciferin, con
  * replays alike.  enable != 0 starts recording; read returns totals in
  * milliseconds and step counts since the last enable: times[0] the scoring
  * launches, times[1] the rest of the step (the FP32 form's per-glowworm
- * epilogue, selection + movement). */
+ * epilogue, selection + movement), times[2] the ligand displacement GEMM
+ * (0 when it is off); times_ms[3], launches[3]. */
 int sdgso_run_profile(sdgso_run r, int enable);
 int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches);
 
 /* Guard-band statistics of the FP32 form since the last enable of profiling:
  * stats[0] pairs recomputed in FP64. */
 int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
+
+/* The ligand's normal-mode displacement as an FP32 matrix-core GEMM,
+ * (poses x modes) . (modes x 3 Nl), the form the reference's GPU path
+ * materialises per pose (prepare_gso_task.cu:268-333; the CPU path adds the
+ * modes per atom, prepare_gso_tasks.cc:487-504).  When it is on, an FP32 run
+ * with ligand modes runs it before each scoring pass, by
+ * v_mfma_f32_32x32x2f32 over every glowworm of the run, and the scoring
+ * kernels read its output instead of summing the ligand modes themselves; it
+ * needs 768 bytes of device memory per (glowworm, 64-atom ligand block).  A
+ * run starts with it off.  sdgso_run_tune times a scoring pass of the chosen
+ * kernel variants with it (the GEMM included) and without it and keeps the
+ * faster (it stays off when its output does not fit in device memory);
+ * sdgso_run_set_ligand_gemm turns it on (SDGSO_ENOMEM when its output does
+ * not fit) or off (enable = 0); the FP64 form and runs without ligand modes
+ * never use it.  sdgso_run_ligand_gemm reports whether it is on, the bytes it
+ * holds and tune_ms[2], the last tune's milliseconds per scoring pass with
+ * and without it (zeros before a tune); any pointer may be NULL.
+ * sdgso_run_ligand_displacements runs it on the current poses and returns
+ * the FP32 displacements of glowworms [first, first + count) of one complex,
+ * out[count][Nl][3] by original atom (SDGSO_ESTATE when it is off). */
+int sdgso_run_set_ligand_gemm(sdgso_run r, int enable);
+int sdgso_run_ligand_gemm(sdgso_run r, int *enabled, int64_t *bytes, double *tune_ms);
+int sdgso_run_ligand_displacements(sdgso_run r, int first, int count, float *out);
 
 /* Autotuner of the FP32 guard-band scoring kernel (no counterpart in the
  * reference, whose tile sizes are compile-time constants,

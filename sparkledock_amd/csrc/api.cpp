@@ -236,9 +236,22 @@ struct sdgso_run_s {
     // of the list of block pairs that two-block variants read)
     std::vector<int> variants;
     std::vector<size_t> work0, work2_0;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_score, ev_move;
-    double t_score = 0, t_move = 0;
-    int64_t n_score = 0, n_move = 0;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_score, ev_move, ev_lgemm;
+    double t_score = 0, t_move = 0, t_lgemm = 0;
+    int64_t n_score = 0, n_move = 0, n_lgemm = 0;
+    // FP32 form with ligand modes: the ligand displacement GEMM that runs
+    // before each scoring pass when lgemm is set.  ldisp holds its output,
+    // [glowworm][block][3][64] f32 (complex k's glowworm gid at
+    // ldisp_base[k] + gid * nlblocks_k * 192); the GEMM's items of complex k
+    // are [lg_item0[k], lg_item0[k + 1]) over glowworms [lg_glow0[k],
+    // lg_glow0[k + 1]).
+    bool lgemm = false;
+    DevBuf<float> ldisp;
+    DevBuf<long long> ldisp_base, lg_item0;
+    DevBuf<int> lg_glow0;
+    long long lg_items = 0;
+    hipGraphExec_t graph_lgemm = nullptr;  // profiled replays: the GEMM alone
+    double lgemm_tune_ms[2] = {0, 0};      // last tune: scoring pass with, without it
 };
 
 static void check_table_and_types(const double *table, const int *rt, int nr, const int *lt, int nl) {
@@ -450,7 +463,7 @@ extern "C" {
 
 const char *sdgso_last_error(void) { return g_error.c_str(); }
 
-int sdgso_version(void) { return 100; }
+int sdgso_version(void) { return 101; }
 
 void sdgso_default_params(sdgso_params *p) {
     if (!p) return;
@@ -601,7 +614,54 @@ static ScoreArgs score_args(sdgso_run r) {
     a.work = r->work.p;
     a.n_work = (int)r->work.n;
     a.counter = r->counter.p;
+    a.ldisp = r->lgemm ? r->ldisp.p : nullptr;
+    a.ldisp_base = r->ldisp_base.p;
     return a;
+}
+
+// The ligand displacement GEMM over every glowworm of the run (FP32 form,
+// when enabled): it reads the current poses, so it runs before each scoring
+// pass over them.
+static void enqueue_lgemm(sdgso_run r) {
+    if (!r->lgemm) return;
+    LigDispArgs g{};
+    g.cx = r->dcx.p;
+    g.n_cx = r->n_complex;
+    g.pos = r->pos.p;
+    g.P = r->P;
+    g.nmr = r->nmr;
+    g.item0 = r->lg_item0.p;
+    g.glow0 = r->lg_glow0.p;
+    g.ldisp = r->ldisp.p;
+    g.ldisp_base = r->ldisp_base.p;
+    g.n_items = r->lg_items;
+    HIPCHK(launch_ligand_disp(g, r->stream_h));
+}
+
+// Switches the ligand displacement GEMM on (allocating its output) or off
+// (freeing it).  Runs without ligand modes, and the FP64 form, never use it.
+static void set_lgemm(sdgso_run r, bool on) {
+    r->lgemm = false;
+    r->ldisp.reset();
+    if (!on || r->form != SDGSO_FP32_GUARD || r->nml == 0) return;
+    std::vector<long long> base, item0(1, 0);
+    std::vector<int> glow0(1, 0);
+    size_t floats = 0;
+    for (int k = 0; k < r->n_complex; k++) {
+        const long long nb = r->cxs[k]->dev.nlblocks;
+        const long long ng = (long long)r->n_swarms[k] * r->G;
+        base.push_back((long long)floats - (long long)glow0.back() * nb * 3 * kBlock);
+        floats += (size_t)(ng * nb * 3 * kBlock);
+        item0.push_back(item0.back() + (ng + 31) / 32 * nb);
+        glow0.push_back(glow0.back() + (int)ng);
+    }
+    if (item0.back() > 0x7fffffffLL) fail(SDGSO_EINVAL, "too many ligand GEMM items");
+    r->ldisp.alloc(floats);
+    r->ldisp_base.upload(base);
+    r->lg_item0.upload(item0);
+    r->lg_glow0.upload(glow0);
+    r->lg_items = item0.back();
+    r->lgemm = true;
 }
 
 // The work of complexes [k0, k1) in the list that variant v reads: (glowworm,
@@ -634,6 +694,7 @@ static void enqueue_score(sdgso_run r) {
     if (r->form == SDGSO_FP64) {
         HIPCHK(launch_score64(a, r->max_nl, r->stream_h));
     } else {
+        enqueue_lgemm(r);
         launch_score32_run(r, a);
         HIPCHK(launch_finish32(a, r->stream_h));
     }
@@ -668,7 +729,8 @@ static void enqueue_select_move(sdgso_run r, int do_select, int do_move) {
     HIPCHK(launch_select_move(m, r->stream_h));
 }
 
-// A step in two parts: the scoring launches, and the rest (the FP32 form's
+// A step in three parts: the ligand displacement GEMM (FP32 form, when
+// enabled), the scoring launches, and the rest (the FP32 form's
 // per-glowworm epilogue, selection + movement, the device step counter).
 static void enqueue_score_launches(sdgso_run r, const ScoreArgs &a) {
     if (r->form == SDGSO_FP64)
@@ -683,17 +745,22 @@ static void enqueue_rest(sdgso_run r, const ScoreArgs &a) {
     HIPCHK(launch_advance(r->d_k.p, r->stream_h));
 }
 
-// Events around the two parts of the next step (profiling).
-static void step_events(sdgso_run r, std::pair<hipEvent_t, hipEvent_t> &e1,
+// Events around the parts of the next step (profiling).
+static void step_events(sdgso_run r, std::pair<hipEvent_t, hipEvent_t> &e0,
+                        std::pair<hipEvent_t, hipEvent_t> &e1,
                         std::pair<hipEvent_t, hipEvent_t> &e2) {
+    if (r->lgemm) profile_pair(r, r->ev_lgemm, e0);
     profile_pair(r, r->ev_score, e1);
     profile_pair(r, r->ev_move, e2);
 }
 
 static void enqueue_step(sdgso_run r, bool timed) {
-    std::pair<hipEvent_t, hipEvent_t> e1{}, e2{};
-    if (timed) step_events(r, e1, e2);
+    std::pair<hipEvent_t, hipEvent_t> e0{}, e1{}, e2{};
+    if (timed) step_events(r, e0, e1, e2);
     const ScoreArgs a = score_args(r);
+    if (timed && r->lgemm) HIPCHK(hipEventRecord(e0.first, r->stream_h));
+    enqueue_lgemm(r);
+    if (timed && r->lgemm) HIPCHK(hipEventRecord(e0.second, r->stream_h));
     if (timed) HIPCHK(hipEventRecord(e1.first, r->stream_h));
     enqueue_score_launches(r, a);
     if (timed) {
@@ -705,15 +772,17 @@ static void enqueue_step(sdgso_run r, bool timed) {
 }
 
 // Captures part of a step (0: all of it, 1: the scoring launches, 2: the
-// rest) as an executable graph.  The stream index lives on the device
-// (advanced by the last node), so every replay is the same graph.
+// rest, 3: the ligand displacement GEMM) as an executable graph.  The stream
+// index lives on the device (advanced by the last node), so every replay is
+// the same graph.
 static hipGraphExec_t capture_step(sdgso_run r, int part) {
     hipGraph_t g = nullptr;
     HIPCHK(hipStreamBeginCapture(r->stream_h, hipStreamCaptureModeThreadLocal));
     try {
         const ScoreArgs a = score_args(r);
-        if (part != 2) enqueue_score_launches(r, a);
-        if (part != 1) enqueue_rest(r, a);
+        if (part == 0 || part == 3) enqueue_lgemm(r);
+        if (part == 0 || part == 1) enqueue_score_launches(r, a);
+        if (part == 0 || part == 2) enqueue_rest(r, a);
     } catch (...) {
         // leave the stream out of capture mode before reporting
         if (hipStreamEndCapture(r->stream_h, &g) == hipSuccess && g) (void)hipGraphDestroy(g);
@@ -728,13 +797,21 @@ static hipGraphExec_t capture_step(sdgso_run r, int part) {
 }
 
 static void drop_graph(sdgso_run r) {
-    for (hipGraphExec_t *g : {&r->graph, &r->graph_score, &r->graph_rest}) {
+    for (hipGraphExec_t *g : {&r->graph, &r->graph_score, &r->graph_rest, &r->graph_lgemm}) {
         if (*g) (void)hipGraphExecDestroy(*g);
         *g = nullptr;
     }
 }
 
 static void harvest_profile(sdgso_run r) {
+    for (auto &e : r->ev_lgemm) {
+        float ms = 0;
+        HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
+        r->t_lgemm += ms;
+        r->n_lgemm++;
+        (void)hipEventDestroy(e.first);
+        (void)hipEventDestroy(e.second);
+    }
     for (auto &e : r->ev_score) {
         float ms = 0;
         HIPCHK(hipEventElapsedTime(&ms, e.first, e.second));
@@ -751,6 +828,7 @@ static void harvest_profile(sdgso_run r) {
         (void)hipEventDestroy(e.first);
         (void)hipEventDestroy(e.second);
     }
+    r->ev_lgemm.clear();
     r->ev_score.clear();
     r->ev_move.clear();
 }
@@ -891,6 +969,10 @@ int sdgso_run_destroy(sdgso_run r) {
         if (g_device >= 0) (void)hipSetDevice(g_device);
         if (r->stream_h) (void)hipStreamSynchronize(r->stream_h);
         drop_graph(r);
+        for (auto &e : r->ev_lgemm) {
+            (void)hipEventDestroy(e.first);
+            (void)hipEventDestroy(e.second);
+        }
         for (auto &e : r->ev_score) {
             (void)hipEventDestroy(e.first);
             (void)hipEventDestroy(e.second);
@@ -931,13 +1013,20 @@ int sdgso_run_steps(sdgso_run r, int n, int use_graph) {
                 }
                 return;
             }
-            // profiled replays: the step as two graphs with the timing events
-            // between them (fresh events per step, read back later)
+            // profiled replays: the step as two graphs (three with the
+            // ligand displacement GEMM) with the timing events between them
+            // (fresh events per step, read back later)
+            if (r->lgemm && !r->graph_lgemm) r->graph_lgemm = capture_step(r, 3);
             if (!r->graph_score) r->graph_score = capture_step(r, 1);
             if (!r->graph_rest) r->graph_rest = capture_step(r, 2);
             for (int i = 0; i < n; i++) {
-                std::pair<hipEvent_t, hipEvent_t> e1{}, e2{};
-                step_events(r, e1, e2);
+                std::pair<hipEvent_t, hipEvent_t> e0{}, e1{}, e2{};
+                step_events(r, e0, e1, e2);
+                if (r->lgemm) {
+                    HIPCHK(hipEventRecord(e0.first, r->stream_h));
+                    HIPCHK(hipGraphLaunch(r->graph_lgemm, r->stream_h));
+                    HIPCHK(hipEventRecord(e0.second, r->stream_h));
+                }
                 HIPCHK(hipEventRecord(e1.first, r->stream_h));
                 HIPCHK(hipGraphLaunch(r->graph_score, r->stream_h));
                 HIPCHK(hipEventRecord(e1.second, r->stream_h));
@@ -1022,8 +1111,8 @@ int sdgso_run_profile(sdgso_run r, int enable) {
         if (!r) fail(SDGSO_EINVAL, "null run");
         HIPCHK(hipStreamSynchronize(r->stream_h));
         harvest_profile(r);
-        r->t_score = r->t_move = 0;
-        r->n_score = r->n_move = 0;
+        r->t_score = r->t_move = r->t_lgemm = 0;
+        r->n_score = r->n_move = r->n_lgemm = 0;
         r->profiling = enable != 0;
         HIPCHK(hipMemset(r->guard_count.p, 0, sizeof(unsigned long long) * 4));
     });
@@ -1037,10 +1126,12 @@ int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches) {
         if (times_ms) {
             times_ms[0] = r->t_score;
             times_ms[1] = r->t_move;
+            times_ms[2] = r->t_lgemm;
         }
         if (launches) {
             launches[0] = r->n_score;
             launches[1] = r->n_move;
+            launches[2] = r->n_lgemm;
         }
     });
 }
@@ -1052,6 +1143,65 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats) {
         unsigned long long h[4];
         HIPCHK(hipMemcpy(h, r->guard_count.p, sizeof(h), hipMemcpyDeviceToHost));
         stats[0] = (int64_t)h[0];
+    });
+}
+
+static int complex_of(sdgso_run r, int gid, int *first);
+
+int sdgso_run_set_ligand_gemm(sdgso_run r, int enable) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        ensure_device();
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        drop_graph(r);
+        set_lgemm(r, enable != 0);
+    });
+}
+
+int sdgso_run_ligand_gemm(sdgso_run r, int *enabled, int64_t *bytes, double *tune_ms) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (enabled) *enabled = r->lgemm ? 1 : 0;
+        if (bytes) *bytes = (int64_t)(r->ldisp.n * sizeof(float));
+        if (tune_ms) {
+            tune_ms[0] = r->lgemm_tune_ms[0];
+            tune_ms[1] = r->lgemm_tune_ms[1];
+        }
+    });
+}
+
+int sdgso_run_ligand_displacements(sdgso_run r, int first, int count, float *out) {
+    return guarded([&] {
+        if (!r || !out) fail(SDGSO_EINVAL, "null argument");
+        if (!r->lgemm) fail(SDGSO_ESTATE, "the run's ligand displacement GEMM is off");
+        if (count <= 0) return;
+        if (first < 0 || first + count > r->N) fail(SDGSO_EINVAL, "glowworm range out of the run");
+        int g0 = 0;
+        const int k = complex_of(r, first, &g0);
+        if (complex_of(r, first + count - 1, nullptr) != k)
+            fail(SDGSO_EINVAL, "glowworm range spans two complexes");
+        ensure_device();
+        enqueue_lgemm(r);
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        const sdgso_complex c = r->cxs[k];
+        const int nb = c->dev.nlblocks, nl = c->nl;
+        std::vector<int> perm((size_t)nb * kBlock);
+        HIPCHK(hipMemcpy(perm.data(), c->lperm.p, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
+        long long base = 0;
+        HIPCHK(hipMemcpy(&base, r->ldisp_base.p + k, sizeof(base), hipMemcpyDeviceToHost));
+        const size_t per = (size_t)nb * 3 * kBlock;
+        std::vector<float> h(per * count);
+        HIPCHK(hipMemcpy(h.data(), r->ldisp.p + base + (long long)first * (long long)per,
+                         h.size() * sizeof(float), hipMemcpyDeviceToHost));
+        for (int g = 0; g < count; g++)
+            for (int b = 0; b < nb; b++)
+                for (int sl = 0; sl < kBlock; sl++) {
+                    const int atom = perm[(size_t)b * kBlock + sl];
+                    if (atom < 0 || atom >= nl) continue;
+                    for (int comp = 0; comp < 3; comp++)
+                        out[((size_t)g * nl + atom) * 3 + comp] =
+                            h[g * per + (size_t)b * 3 * kBlock + comp * kBlock + sl];
+                }
     });
 }
 
@@ -1077,6 +1227,7 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
         // and step are left alone (the next step rescores these poses anyway)
         ScoreArgs a = score_args(r);
         a.update_luciferin = 0;
+        enqueue_lgemm(r);  // the ligand displacements of these poses, for every launch below
         hipEvent_t e0, e1;
         HIPCHK(hipEventCreate(&e0));
         HIPCHK(hipEventCreate(&e1));
@@ -1102,6 +1253,43 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
             }
             r->variants[k] = best_v;
             if (chosen) chosen[k] = best_v;
+        }
+        // the ligand displacement GEMM: a scoring pass of the chosen variants
+        // with it (the GEMM included) and one without it; the faster stays
+        r->lgemm_tune_ms[0] = r->lgemm_tune_ms[1] = 0;
+        if (r->nml > 0) {
+            const bool was = r->lgemm;
+            bool can = true;
+            try {
+                if (!was) set_lgemm(r, true);
+            } catch (const Failure &f) {
+                // its output does not fit in device memory: it stays off
+                if (f.code != SDGSO_ENOMEM) throw;
+                (void)hipGetLastError();  // the failed allocation's error
+                set_lgemm(r, false);
+                can = false;
+            }
+            if (can) {
+                double t[2] = {0, 0};
+                for (int pass = 0; pass < 2; pass++) {
+                    ScoreArgs b = score_args(r);
+                    b.update_luciferin = 0;
+                    if (pass == 1) b.ldisp = nullptr;
+                    for (int i = 0; i <= repeats; i++) {  // i = 0: warm-up
+                        if (i == 1) HIPCHK(hipEventRecord(e0, r->stream_h));
+                        if (pass == 0) enqueue_lgemm(r);
+                        launch_score32_run(r, b);
+                    }
+                    HIPCHK(hipEventRecord(e1, r->stream_h));
+                    HIPCHK(hipEventSynchronize(e1));
+                    float ms = 0;
+                    HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                    t[pass] = ms / repeats;
+                }
+                r->lgemm_tune_ms[0] = t[0];
+                r->lgemm_tune_ms[1] = t[1];
+                if (t[1] <= t[0]) set_lgemm(r, false);
+            }
         }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
@@ -1144,6 +1332,7 @@ int sdgso_run_filter_stats(sdgso_run r, int64_t *stats) {
         a.guard_count = nullptr;
         a.stats = st.p;
         // the per-block partials are scratch: no epilogue, the run state stays as it was
+        enqueue_lgemm(r);
         launch_score32_run(r, a);
         HIPCHK(hipStreamSynchronize(r->stream_h));
         HIPCHK(hipMemcpy(h.data(), st.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost));
@@ -1407,7 +1596,9 @@ int sdgso_score_poses(sdgso_complex c, int n_poses, const double *poses, int for
         if (n_poses < 0 || (n_poses > 0 && (!poses || !out_score)))
             fail(SDGSO_EINVAL, "bad pose buffers");
         if (n_poses == 0) return;
-        // a run of n_poses single-glowworm swarms, scored once without luciferin
+        // a run of n_poses single-glowworm swarms, scored once without
+        // luciferin; the scoring set-up sums the ligand modes itself (no
+        // displacement GEMM), as in sdgso_score_poses_noalloc
         sdgso_run r = nullptr;
         sdgso_complex cs[1] = {c};
         int ns[1] = {n_poses};

@@ -153,21 +153,23 @@ __device__ __forceinline__ void ligand_atom64(const DevComplex &c, const double 
 
 // The same for slot `lane` of ligand block b, read from the block-slot copy
 // (every load of a wave is contiguous), for a position that the FP32 kernels
-// round to their grid: the mode displacement is an FP32 FMA chain in mode
-// order over FP32 modes and extents (within (nml + 2) u S of the exact one,
-// u = 2^-24 and S the pose's ligand displacement bound), added to the FP64
-// coordinates, which the pose's matrix R then rotates.
+// round to their grid: the mode displacement, FP32, is added to the FP64
+// coordinates, which the pose's matrix R then rotates.  The displacement is
+// d (have_d: this slot's values from the run's ligand displacement GEMM) or
+// formed here as an FP32 FMA chain in mode order over the FP32 modes and
+// extents.  Either is within (nml + 2) u S of the exact one, u = 2^-24 and S
+// the pose's ligand displacement bound.
 __device__ __forceinline__ void ligand_slot_grid(const DevComplex &c, const double *pose,
-                                                 const Rot64 &R, int b, int lane, double &x,
-                                                 double &y, double &z) {
-    const int nm = c.nml;
+                                                 const Rot64 &R, int b, int lane, bool have_d,
+                                                 float3 d, double &x, double &y, double &z) {
+    const int nm = have_d ? 0 : c.nml;
     const double *B = c.lblk + (size_t)b * 3 * kBlock + lane;
     const float *BM = c.lblkm + (size_t)b * 3 * nm * kBlock + lane;
     x = gld(B, 0);
     y = gld(B, kBlock);
     z = gld(B, 2 * kBlock);
     const double *ext = pose + 7 + c.nmr;
-    float dx = 0.f, dy = 0.f, dz = 0.f;
+    float dx = d.x, dy = d.y, dz = d.z;
     int m = 0;
     for (; m + 4 <= nm; m += 4) {
         float mx[4], my[4], mz[4], e[4];

@@ -211,10 +211,12 @@ __device__ __forceinline__ int scan_candidates(const DevComplex &c, const WaveCt
 
 // Set-up common to the variants: pose, block sphere, ligand atom, receptor
 // extents (LDS), candidate tiles (LDS; left to the caller when scan is
-// false).  Returns false for an idle wave.
+// false).  have_d: dA holds this lane's slot of block b's ligand
+// displacement from the run's GEMM.  Returns false for an idle wave.
 __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex &c, int gid,
-                                           int b, float *ef, unsigned short *ct, WaveCtx &x,
-                                           int cap = kMaxTiles, bool scan = true) {
+                                           int b, bool have_d, float3 dA, float *ef,
+                                           unsigned short *ct, WaveCtx &x, int cap = kMaxTiles,
+                                           bool scan = true) {
     x.lane = threadIdx.x & 63;
     x.gid = gid;
     x.b = b;
@@ -235,7 +237,7 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     x.L = make_int4(-kFar, -kFar, -kFar, 0);
     if (x.lig >= 0) {
         double px, py, pz;
-        ligand_slot_grid(c, x.pose, x.rot, b, x.lane, px, py, pz);
+        ligand_slot_grid(c, x.pose, x.rot, b, x.lane, have_d, dA, px, py, pz);
         x.L = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
                         gld(c.ltype_b, x.lig));
     }
@@ -252,11 +254,15 @@ __device__ __forceinline__ bool wave_setup(const ScoreArgs &a, const DevComplex 
     // errors k u S of both atoms, u = 2^-24, S the displacement bound (dr,
     // dl) and k the roundings per mode term (FP32 mode and extent, and the
     // sum: at most 11 for the broadcast variant's in-lane FMAs and group sum,
-    // nm + 2 for a chain in mode order, as the ligand's).  With |dx| + |dy| +
-    // |dz| <= 15 sqrt(3) and 3 u 225 for the FP32 sum of squares,
+    // nm + 2 for a sum of nm products in any order, as the ligand's).  The
+    // ligand's from the run's matrix-core GEMM takes 2 (nm + 2): the bound
+    // holds for any order of correctly rounded steps, and the factor covers
+    // accumulation steps that round less tightly.  With |dx| + |dy| + |dz| <=
+    // 15 sqrt(3) and 3 u 225 for the FP32 sum of squares,
     // |d2 - d2_ref| <= 9.04e-5 + 3.098e-6 (k_r dr + k_l dl) A^2.
+    const int k_l = have_d ? 2 * c.nml + 4 : c.nml + 2;
     x.band = fmaxf(a.band, 9.1e-5f + 3.11e-6f * ((float)max(11, c.nmr + 2) * dr +
-                                                   (float)(c.nml + 2) * dl));
+                                                   (float)k_l * dl));
     x.dlr = dl + dr;
     x.dl = dl;
     x.reach = 15.0f + blk.w + dl + dr + 0.01f;
@@ -440,7 +446,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     auto item = [&](const int wv) {
     const int gid = __builtin_amdgcn_readfirstlane(a.work[2 * wv]);
     const int b = __builtin_amdgcn_readfirstlane(a.work[2 * wv + 1]);
-    const DevComplex &c = a.cx[__builtin_amdgcn_readfirstlane(a.swarm_cx[gid / a.G])];
+    const int kx = __builtin_amdgcn_readfirstlane(a.swarm_cx[gid / a.G]);
+    const DevComplex &c = a.cx[kx];
     // the fields the tile visits and the evaluation read, in scalar registers
     const int ntiles = __builtin_amdgcn_readfirstlane(c.ntiles);
     const int nmr = __builtin_amdgcn_readfirstlane(c.nmr);
@@ -459,8 +466,22 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     const unsigned q_dummy = (unsigned)(kQueue + (int)(threadIdx.x & 63)) << 2;  // this lane's dummy slot (bytes)
     float *ef = s_ef[w];
     unsigned short *ct = s_ct[w];
+    // this lane's slots of the wave's ligand blocks in the run's ligand
+    // displacement GEMM output, if any, loaded first: they come from HBM
+    const bool have_d = a.ldisp != nullptr;
+    float3 dA = make_float3(0.f, 0.f, 0.f), dB = dA;
+    if (have_d) {
+        const int nlb = __builtin_amdgcn_readfirstlane(c.nlblocks);
+        const float *pA = uniform_ptr(a.ldisp + a.ldisp_base[kx] +
+                                      ((long long)gid * nlb + b) * 3 * kBlock);
+        const int l = threadIdx.x & 63;
+        dA = make_float3(gld(pA, l), gld(pA, kBlock + l), gld(pA, 2 * kBlock + l));
+        if (kLB == 2 && b + 1 < nlb)
+            dB = make_float3(gld(pA, 3 * kBlock + l), gld(pA, 4 * kBlock + l),
+                             gld(pA, 5 * kBlock + l));
+    }
     WaveCtx x;
-    wave_setup(a, c, gid, b, ef, ct, x, kCandB, kLB == 1);
+    wave_setup(a, c, gid, b, have_d, dA, ef, ct, x, kCandB, kLB == 1);
     const int lane = x.lane;
     lg[lane] = x.L;
     // second block (kLB = 2): b + 1 when the glowworm has it; its sphere (fb,
@@ -483,7 +504,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             lig_b = gld(c.lperm, b2 * kBlock + lane);
             if (lig_b >= 0) {
                 double px, py, pz;
-                ligand_slot_grid(c, x.pose, x.rot, b2, lane, px, py, pz);
+                ligand_slot_grid(c, x.pose, x.rot, b2, lane, have_d, dB, px, py, pz);
                 LB = make_int4(qround(px - x.cb[0]), qround(py - x.cb[1]), qround(pz - x.cb[2]),
                                gld(c.ltype_b, lig_b));
             }

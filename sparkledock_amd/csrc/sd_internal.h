@@ -25,10 +25,13 @@
 //     lhalf   f32x4 [nlblocks]     half extents of the block's box about that centre
 //     lblk    f64 [nlblocks][3][64]        coordinates of the block slots
 //     lblkm   f32 [nlblocks][3*nml][64]    modes of the block slots (FP32)
+//       (row 3*m + c: coordinate c of mode m)
 //   run state, N = sum(swarms) * G glowworms
 //     pos [N][P] f64, luciferin/score/vision [N] f64, ref [N][3] f64,
 //     nnei/moved/select/step [N] i32, pairs [N] i64,
 //     stream [sum S][n_steps][G] f64
+//     ldisp  f32, per glowworm [nlblocks][3][64]: the ligand's FP32 mode
+//            displacement by block slot (FP32 form, ligand modes > 0)
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -148,6 +151,28 @@ struct ScoreArgs {
     unsigned long long *stats;
     // FP32 form, work-queue variants: the queue's counter (one word)
     unsigned *counter;
+    // FP32 form, optional: the ligand's FP32 mode displacement of every
+    // (glowworm, ligand block slot) from ligand_disp_kernel, [3][64] floats
+    // per (glowworm, block); glowworm gid of complex k (a.cx[k], nlb blocks)
+    // starts at ldisp + ldisp_base[k] + gid * nlb * 192.  Null: the scoring
+    // set-up sums the FP32 ligand modes itself.
+    const float *ldisp;
+    const long long *ldisp_base;
+};
+
+// The ligand displacement GEMM of a run: for every complex k of the run,
+// items [item0[k], item0[k + 1]) are (32-glowworm tile, ligand block) pairs,
+// tile-major, over the complex's glowworms [glow0[k], glow0[k + 1]).
+struct LigDispArgs {
+    const DevComplex *cx;   // [n_cx]
+    int n_cx;
+    const double *pos;      // [N][P] run poses
+    int P, nmr;
+    const long long *item0; // [n_cx + 1]
+    const int *glow0;       // [n_cx + 1]
+    float *ldisp;           // as ScoreArgs::ldisp
+    const long long *ldisp_base;
+    long long n_items;
 };
 
 struct SelectMoveArgs {
@@ -191,6 +216,7 @@ const char *score32_variant_name(int v);
 int score32_variant_blocks(int v);  // ligand blocks per wave: 1, or 2 (work list of block pairs)
 int score32_default_variant();
 hipError_t launch_finish32(const ScoreArgs &a, hipStream_t s);
+hipError_t launch_ligand_disp(const LigDispArgs &a, hipStream_t s);
 hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s);
 hipError_t launch_advance(int *d_k, hipStream_t s);
 hipError_t launch_coords(const CoordsArgs &a, int count, hipStream_t s);

@@ -68,6 +68,9 @@ SIGNATURES = [
     ("sdgso_run_profile", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_run_profile_read", ctypes.c_int, [_VP, _D, _I64]),
     ("sdgso_run_guard_stats", ctypes.c_int, [_VP, _I64]),
+    ("sdgso_run_set_ligand_gemm", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("sdgso_run_ligand_gemm", ctypes.c_int, [_VP, _I, _I64, _D]),
+    ("sdgso_run_ligand_displacements", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _F]),
     ("sdgso_tune_variants", ctypes.c_int, []),
     ("sdgso_tune_variant_name", ctypes.c_char_p, [ctypes.c_int]),
     ("sdgso_run_tune", ctypes.c_int, [_VP, ctypes.c_int, _D, _I]),
@@ -275,10 +278,40 @@ class Run:
         _check(lib().sdgso_run_profile(self.h, 1 if enable else 0))
 
     def profile_read(self):
-        t = np.zeros(2)
-        n = np.zeros(2, np.int64)
+        t = np.zeros(3)
+        n = np.zeros(3, np.int64)
         _check(lib().sdgso_run_profile_read(self.h, _p(t), _p(n, _I64)))
-        return dict(score_ms=t[0], move_ms=t[1], score_launches=int(n[0]), move_launches=int(n[1]))
+        return dict(score_ms=t[0], move_ms=t[1], lgemm_ms=t[2], score_launches=int(n[0]),
+                    move_launches=int(n[1]), lgemm_launches=int(n[2]))
+
+    def set_ligand_gemm(self, enable):
+        """Switch the ligand displacement GEMM (FP32 form) on or off; a run
+        starts with it off and the tuner keeps it when it is faster."""
+        _check(lib().sdgso_run_set_ligand_gemm(self.h, 1 if enable else 0))
+
+    def ligand_gemm(self):
+        """(on, bytes of its output) of the ligand displacement GEMM."""
+        e = np.zeros(1, np.int32)
+        b = np.zeros(1, np.int64)
+        _check(lib().sdgso_run_ligand_gemm(self.h, _p(e, _I), _p(b, _I64), None))
+        return bool(e[0]), int(b[0])
+
+    def ligand_gemm_tune_ms(self):
+        """The last tune's ms per scoring pass with and without the GEMM."""
+        t = np.zeros(2)
+        _check(lib().sdgso_run_ligand_gemm(self.h, None, None, _p(t)))
+        return float(t[0]), float(t[1])
+
+    def ligand_displacements(self, first, count):
+        """The GEMM's FP32 ligand displacements of glowworms [first, first +
+        count) (one complex) for the current poses: [count][Nl][3]."""
+        k, g0 = 0, 0
+        while first >= g0 + self.n_swarms[k] * self.G:
+            g0 += self.n_swarms[k] * self.G
+            k += 1
+        out = np.zeros((count, self.complexes[k].nl, 3), np.float32)
+        _check(lib().sdgso_run_ligand_displacements(self.h, int(first), int(count), _p(out, _F)))
+        return out
 
     def tune(self, repeats=1):
         """Time every FP32 scoring-kernel variant on the current poses of each
