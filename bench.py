@@ -547,6 +547,9 @@ def main():
         "ligand_gemm": lgemm,
         "anm_gemm_probe": anm,
         "autotune": {"chosen_per_complex": chosen_names,
+                     "work_order": {"xcd_chunk": run.work_order()[0],
+                                    "tuner_ms_per_scoring_pass": dict(zip(
+                                        ("list_order", "xcd_chunked"), run.work_order()[1]))},
                      "ms_per_launch": ({k: sum(v) for k, v in tune_ms.items()} if tune_ms else None)},
     }
     print(json.dumps(out), flush=True)

@@ -169,6 +169,19 @@ int sdgso_run_profile_read(sdgso_run r, double *times_ms, int64_t *launches);
  * stats[0] pairs recomputed in FP64. */
 int sdgso_run_guard_stats(sdgso_run r, int64_t *stats);
 
+/* Work order of the FP32 scoring launches (no counterpart in the reference):
+ * xcd_chunk = 0 hands the (glowworm, ligand block) work items to waves in
+ * list order; xcd_chunk = K > 0 in chunks of K items per XCD of the MI355X
+ * (workgroup w runs on XCD w % 8, so XCD x takes chunks x, x + 8, ...: the
+ * receptor region that one XCD's L2 holds is that of fewer swarms).  Results
+ * do not depend on it.  A run starts in list order; sdgso_run_tune times a
+ * scoring pass of the chosen variants in each order (K = 8192) and keeps the
+ * faster.  sdgso_run_work_order returns the order and tune_ms[2], the last
+ * tune's milliseconds per pass in list order and chunked (either pointer may
+ * be NULL). */
+int sdgso_run_set_work_order(sdgso_run r, int xcd_chunk);
+int sdgso_run_work_order(sdgso_run r, int *xcd_chunk, double *tune_ms);
+
 /* The ligand's normal-mode displacement as an FP32 matrix-core GEMM,
  * (poses x modes) . (modes x 3 Nl), the form the reference's GPU path
  * materialises per pose (prepare_gso_task.cu:268-333; the CPU path adds the

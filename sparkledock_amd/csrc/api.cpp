@@ -252,7 +252,15 @@ struct sdgso_run_s {
     long long lg_items = 0;
     hipGraphExec_t graph_lgemm = nullptr;  // profiled replays: the GEMM alone
     double lgemm_tune_ms[2] = {0, 0};      // last tune: scoring pass with, without it
+    // FP32 form: the scoring launches' work order (ScoreArgs::xcd_chunk), set
+    // by the tuner; its last timing of a scoring pass in list order and
+    // XCD-chunked
+    int xcd_chunk = 0;
+    double order_tune_ms[2] = {0, 0};
 };
+
+// The chunk of work items per XCD that the tuner tries against list order.
+constexpr int kXcdChunk = 8192;
 
 static void check_table_and_types(const double *table, const int *rt, int nr, const int *lt, int nl) {
     if (!table) fail(SDGSO_EINVAL, "table is null");
@@ -616,6 +624,7 @@ static ScoreArgs score_args(sdgso_run r) {
     a.counter = r->counter.p;
     a.ldisp = r->lgemm ? r->ldisp.p : nullptr;
     a.ldisp_base = r->ldisp_base.p;
+    a.xcd_chunk = r->xcd_chunk;
     return a;
 }
 
@@ -1148,6 +1157,27 @@ int sdgso_run_guard_stats(sdgso_run r, int64_t *stats) {
 
 static int complex_of(sdgso_run r, int gid, int *first);
 
+int sdgso_run_set_work_order(sdgso_run r, int xcd_chunk) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (xcd_chunk < 0 || xcd_chunk > (1 << 20)) fail(SDGSO_EINVAL, "xcd_chunk out of [0, 2^20]");
+        HIPCHK(hipStreamSynchronize(r->stream_h));
+        drop_graph(r);
+        r->xcd_chunk = xcd_chunk;
+    });
+}
+
+int sdgso_run_work_order(sdgso_run r, int *xcd_chunk, double *tune_ms) {
+    return guarded([&] {
+        if (!r) fail(SDGSO_EINVAL, "null run");
+        if (xcd_chunk) *xcd_chunk = r->xcd_chunk;
+        if (tune_ms) {
+            tune_ms[0] = r->order_tune_ms[0];
+            tune_ms[1] = r->order_tune_ms[1];
+        }
+    });
+}
+
 int sdgso_run_set_ligand_gemm(sdgso_run r, int enable) {
     return guarded([&] {
         if (!r) fail(SDGSO_EINVAL, "null run");
@@ -1253,6 +1283,28 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
             }
             r->variants[k] = best_v;
             if (chosen) chosen[k] = best_v;
+        }
+        // the work order: a scoring pass of the chosen variants in list order
+        // and one XCD-chunked; the faster stays
+        {
+            double t[2] = {0, 0};
+            for (int pass = 0; pass < 2; pass++) {
+                r->xcd_chunk = pass ? kXcdChunk : 0;
+                ScoreArgs b = score_args(r);
+                b.update_luciferin = 0;
+                for (int i = 0; i <= repeats; i++) {  // i = 0: warm-up
+                    if (i == 1) HIPCHK(hipEventRecord(e0, r->stream_h));
+                    launch_score32_run(r, b);
+                }
+                HIPCHK(hipEventRecord(e1, r->stream_h));
+                HIPCHK(hipEventSynchronize(e1));
+                float ms = 0;
+                HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+                t[pass] = ms / repeats;
+            }
+            r->order_tune_ms[0] = t[0];
+            r->order_tune_ms[1] = t[1];
+            r->xcd_chunk = t[1] < t[0] ? kXcdChunk : 0;
         }
         // the ligand displacement GEMM: a scoring pass of the chosen variants
         // with it (the GEMM included) and one without it; the faster stays

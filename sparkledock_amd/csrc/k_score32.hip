@@ -1177,7 +1177,12 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     write_partials(a, x, acc, npairs, nguard, n_cand, n_near, n_queued, b2);
     };
     if constexpr (kPersist == 0) {
-        const int wv = blockIdx.x * kWpb + w;
+        int wv = blockIdx.x * kWpb + w;
+        const int K = a.xcd_chunk;
+        if (K > 0 && wv < a.n_work / (8 * K) * (8 * K)) {
+            const int i = wv >> 3;
+            wv = ((i / K) * 8 + (wv & 7)) * K + i % K;
+        }
         if (wv < a.n_work) item(wv);
     } else if constexpr (kPersist == 1) {
         for (int wv = blockIdx.x * kWpb + w; wv < a.n_work; wv += gridDim.x * kWpb) item(wv);

@@ -158,6 +158,11 @@ struct ScoreArgs {
     // set-up sums the FP32 ligand modes itself.
     const float *ldisp;
     const long long *ldisp_base;
+    // FP32 form, one-item-per-wave variants: 0 takes the work items in list
+    // order; K > 0 hands them out in chunks of K per XCD (workgroup w runs on
+    // XCD w % 8, so the i-th wave an XCD receives takes item (i / K * 8 +
+    // w % 8) * K + i % K; the last 8 K items of a launch stay in order)
+    int xcd_chunk;
 };
 
 // The ligand displacement GEMM of a run: for every complex k of the run,

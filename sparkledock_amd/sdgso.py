@@ -68,6 +68,8 @@ SIGNATURES = [
     ("sdgso_run_profile", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_run_profile_read", ctypes.c_int, [_VP, _D, _I64]),
     ("sdgso_run_guard_stats", ctypes.c_int, [_VP, _I64]),
+    ("sdgso_run_set_work_order", ctypes.c_int, [_VP, ctypes.c_int]),
+    ("sdgso_run_work_order", ctypes.c_int, [_VP, _I, _D]),
     ("sdgso_run_set_ligand_gemm", ctypes.c_int, [_VP, ctypes.c_int]),
     ("sdgso_run_ligand_gemm", ctypes.c_int, [_VP, _I, _I64, _D]),
     ("sdgso_run_ligand_displacements", ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _F]),
@@ -283,6 +285,18 @@ class Run:
         _check(lib().sdgso_run_profile_read(self.h, _p(t), _p(n, _I64)))
         return dict(score_ms=t[0], move_ms=t[1], lgemm_ms=t[2], score_launches=int(n[0]),
                     move_launches=int(n[1]), lgemm_launches=int(n[2]))
+
+    def set_work_order(self, xcd_chunk):
+        """Scoring work order: 0 list order, K > 0 chunks of K items per XCD."""
+        _check(lib().sdgso_run_set_work_order(self.h, int(xcd_chunk)))
+
+    def work_order(self):
+        """(xcd_chunk, (ms per scoring pass in list order, chunked) of the
+        last tune)."""
+        k = np.zeros(1, np.int32)
+        t = np.zeros(2)
+        _check(lib().sdgso_run_work_order(self.h, _p(k, _I), _p(t)))
+        return int(k[0]), (float(t[0]), float(t[1]))
 
     def set_ligand_gemm(self, enable):
         """Switch the ligand displacement GEMM (FP32 form) on or off; a run
