@@ -187,6 +187,7 @@ struct sdgso_complex_s {
     std::vector<float4> h_tiles;
     double lig_rmax = 0;
     DevBuf<double> r0, rmodes, l0, lmodes;
+    DevBuf<double> rmodes_am, lmodes_am;  // the same modes atom-major, [n][nm][3]
     DevBuf<int> rtype, ltype;
     DevBuf<double> table64;    // the caller's table (FP64 form)
     DevBuf<float> table;       // the same rounded to FP32 (FP32 guard form)
@@ -517,6 +518,18 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         c->rmodes.upload(rmsoa);
         c->l0.upload(lsoa);
         c->lmodes.upload(lmsoa);
+        // atom-major copies for the FP32 form's FP64 guard-band recompute: an
+        // atom's modes in consecutive lines
+        auto atom_major = [](const std::vector<double> &msoa, int n, int nm) {
+            std::vector<double> am(msoa.size());
+            for (int m = 0; m < nm; m++)
+                for (int k = 0; k < 3; k++)
+                    for (int i = 0; i < n; i++)
+                        am[((size_t)i * nm + m) * 3 + k] = msoa[((size_t)m * 3 + k) * n + i];
+            return am;
+        };
+        c->rmodes_am.upload(atom_major(rmsoa, n_rec, n_rec_modes));
+        c->lmodes_am.upload(atom_major(lmsoa, n_lig, n_lig_modes));
         std::vector<int> rt(n_rec), lt(n_lig);
         for (int i = 0; i < n_rec; i++) rt[i] = rec_type[i] * SDGSO_NTYPES * SDGSO_NBINS;
         for (int i = 0; i < n_lig; i++) lt[i] = lig_type[i] * SDGSO_NBINS;
@@ -554,6 +567,8 @@ int sdgso_complex_create(int n_rec, const double *rec_xyz, const int *rec_type, 
         d.rmodes = c->rmodes.p;
         d.l0 = c->l0.p;
         d.lmodes = c->lmodes.p;
+        d.rmodes_am = c->rmodes_am.p;
+        d.lmodes_am = c->lmodes_am.p;
         d.rtype = c->rtype.p;
         d.ltype = c->ltype.p;
         d.table = c->table.p;

@@ -67,62 +67,44 @@ constexpr int kFar = 1 << 29;                 // far-away fixed-point coordinate
 // this GPU.  A lane's n terms err by at most n 2^-53 sum |T|, so a sum whose
 // terms cancel keeps its relative accuracy.
 
-// v of lane l (a double) in every lane.
-__device__ __forceinline__ double readlane_f64(double v, int l) {
-    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
-    const unsigned lo = __builtin_amdgcn_readlane((unsigned)u, l);
-    const unsigned hi = __builtin_amdgcn_readlane((unsigned)(u >> 32), l);
-    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+// x/y/z += M[i][m][.] * e_m for m in mode order, one mode at a time (the
+// reference order of prepare_gso_tasks.cc:465-504, as modes_add64), from the
+// atom-major copy of the modes: atom i's values are consecutive, so after the
+// first miss the loads of the later modes hit the cache.
+__device__ __forceinline__ void modes_add64_am(const double *modes_am, int i, const double *ext,
+                                               int nm, double &x, double &y, double &z) {
+    const double *M = modes_am + (size_t)i * nm * 3;
+#pragma unroll 1
+    for (int m = 0; m < nm; m++) {
+        const double e = ext[m];
+        x += gld(M, 3 * m) * e;
+        y += gld(M, 3 * m + 1) * e;
+        z += gld(M, 3 * m + 2) * e;
+    }
 }
 
-// FP64 decision for one pair inside the guard band of a boundary, rebuilt
-// from the pose in the reference's order by the whole wave: returns (FP64
-// table value, 1) for a pair inside (0, 225], else (0, 0), the same in every
-// lane.  lig_atom and rec_atom are wave-uniform.  Lane m < 32 forms the
-// products M_m * e_m of receptor mode m and lane 32 + m those of ligand mode
-// m (each product rounded, as the reference rounds it), so the mode values
-// are fetched in one round trip; the products are then added to the FP64
-// coordinates one mode at a time in mode order (prepare_gso_tasks.cc:465-504),
-// the ligand rotated and translated (:507-526) and the pair term formed
-// (:1542-1566).  The FP32 kernels call it for about 4e-5 of their pairs.
-__device__ __noinline__ double2 guard_pair_wave(const DevComplex &c, const double *pose,
-                                                int lig_atom, int rec_atom) {
+// FP64 decision for a pair inside the guard band of a boundary, rebuilt from
+// the pose in the reference's order: returns (FP64 table value, 1) for a
+// pair inside (0, 225], else (0, 0).  Called from the evaluation loop of the
+// FP32 kernels by the lanes that hold such a pair, for about 4e-5 of the
+// pairs, so it is kept out of line and lean: its mode sums run one mode at a
+// time, which leaves most of the caller's registers untouched across the
+// call.
+__device__ __noinline__ double2 guard_pair64(const DevComplex &c, const double *pose,
+                                             int lig_atom, int rec_atom) {
     if (lig_atom < 0 || rec_atom < 0) return make_double2(0.0, 0.0);
-    const int lane = threadIdx.x & 63, m = lane & 31;
-    const int nmr = c.nmr, nml = c.nml, nr = c.nr, nl = c.nl;
-    double px = 0.0, py = 0.0, pz = 0.0;
-    if (lane < 32 && m < nmr) {
-        const double *M = c.rmodes + (size_t)m * 3 * nr;
-        const double e = gld(pose, 7 + m);
-        px = gld(M, rec_atom) * e;
-        py = gld(M, nr + rec_atom) * e;
-        pz = gld(M, 2 * nr + rec_atom) * e;
-    } else if (lane >= 32 && m < nml) {
-        const double *M = c.lmodes + (size_t)m * 3 * nl;
-        const double e = gld(pose, 7 + nmr + m);
-        px = gld(M, lig_atom) * e;
-        py = gld(M, nl + lig_atom) * e;
-        pz = gld(M, 2 * nl + lig_atom) * e;
-    }
-    double Rx = gld(c.r0, rec_atom), Ry = gld(c.r0, nr + rec_atom), Rz = gld(c.r0, 2 * nr + rec_atom);
-    double Lx = gld(c.l0, lig_atom), Ly = gld(c.l0, nl + lig_atom), Lz = gld(c.l0, 2 * nl + lig_atom);
-    for (int k = 0; k < nmr; k++) {
-        Rx += readlane_f64(px, k);
-        Ry += readlane_f64(py, k);
-        Rz += readlane_f64(pz, k);
-    }
-    for (int k = 0; k < nml; k++) {
-        Lx += readlane_f64(px, 32 + k);
-        Ly += readlane_f64(py, 32 + k);
-        Lz += readlane_f64(pz, 32 + k);
-    }
-    double q[4], inv[4];
-    for (int k = 0; k < 4; k++) q[k] = gld(pose, 3 + k);
-    quat_inverse(q, inv);
-    rotate_point(q, inv, Lx, Ly, Lz);
-    Lx += gld(pose, 0);
-    Ly += gld(pose, 1);
-    Lz += gld(pose, 2);
+    double Rx = gld(c.r0, rec_atom), Ry = gld(c.r0, c.nr + rec_atom),
+           Rz = gld(c.r0, 2 * c.nr + rec_atom);
+    modes_add64_am(c.rmodes_am, rec_atom, pose + 7, c.nmr, Rx, Ry, Rz);
+    double Lx = gld(c.l0, lig_atom), Ly = gld(c.l0, c.nl + lig_atom),
+           Lz = gld(c.l0, 2 * c.nl + lig_atom);
+    modes_add64_am(c.lmodes_am, lig_atom, pose + 7 + c.nmr, c.nml, Lx, Ly, Lz);
+    double inv[4];
+    quat_inverse(pose + 3, inv);
+    rotate_point(pose + 3, inv, Lx, Ly, Lz);
+    Lx += pose[0];
+    Ly += pose[1];
+    Lz += pose[2];
     double v64 = 0.0;
     const bool hit = pair_term64(Rx, Ry, Rz, Lx, Ly, Lz, gld(c.rtype, rec_atom),
                                  gld(c.ltype, lig_atom), c.table, v64);
@@ -653,22 +635,15 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         acc += (double)ring[j];
         ring[j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tbl, off, 0, 0));
         npairs += pc.take ? 1 : 0;
-        // pairs inside a guard band (rare): each decided in FP64 by the
-        // whole wave, its term added by the lane that holds it
-        for (unsigned long long nb = __ballot(pc.near); nb; nb &= nb - 1) {
-            const int k = __builtin_ctzll(nb);
-            const int tk = __builtin_amdgcn_readlane(tag, k);
+        if (pc.near) {
             // ligand slot li of block x.b (li >= 64: slot li - 64 of block x.b + 1)
-            const int li = (int)((unsigned)(tk & 0xffff) >> 4) - w * 64 * kLB;
-            const int ki = (int)((unsigned)tk >> 20) - w * kRtRows;
-            const int lig = __builtin_amdgcn_readfirstlane(gld(c.lperm, x.b * kBlock + li));
-            const int rec = __builtin_amdgcn_readfirstlane(gld(c.rperm, (int)rs[ki]));
-            const double2 g2 = guard_pair_wave(c, x.pose, lig, rec);
-            if (lane == k) {
-                acc += g2.x;
-                npairs += g2.y != 0.0 ? 1 : 0;
-                nguard++;
-            }
+            const int li = (int)((unsigned)(tag & 0xffff) >> 4) - w * 64 * kLB;
+            const int ki = (int)((unsigned)tag >> 20) - w * kRtRows;
+            const double2 g2 = guard_pair64(c, x.pose, gld(c.lperm, x.b * kBlock + li),
+                                            gld(c.rperm, (int)rs[ki]));
+            acc += g2.x;
+            npairs += g2.y != 0.0 ? 1 : 0;
+            nguard++;
         }
     };
     // One round: the 64 queued pairs from entry e0, table load into ring slot j

@@ -6,6 +6,7 @@
 //     rmodes  f64 [nmr][3][nr]     receptor modes, SoA, mask folded in (zeros)
 //     l0      f64 [3][nl]          ligand coordinates, SoA
 //     lmodes  f64 [nml][3][nl]
+//     rmodes_am, lmodes_am  f64 [n][nm][3]: the same, atom-major (guard band)
 //     rtype   i32 [nr]             type * 3360 (row offset into the table)
 //     ltype   i32 [nl]             type * 20
 //     table64 f64 [168*168*20+20]  DFIRE table + 20 zeros (bin 20 of (167,167))
@@ -89,6 +90,8 @@ struct DevComplex {
     const double *rmodes;
     const double *l0;
     const double *lmodes;
+    const double *rmodes_am;  // [nr][nmr][3] the receptor modes atom-major (FP32 form's guard)
+    const double *lmodes_am;  // [nl][nml][3]
     const int *rtype;
     const int *ltype;
     const float *table;      // FP32 copy (FP32 form, its guard-band pairs)
