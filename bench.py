@@ -259,6 +259,11 @@ def main():
     ap.add_argument("--variant", default=None,
                     help="FP32 scoring-kernel variant (name, e.g. bcast2/ring4/occ4, or index) "
                          "to use instead of the autotuner's choice")
+    ap.add_argument("--xcd-chunk", type=int, default=None,
+                    help="scoring work order (0 list order, K chunks of K items per XCD) instead "
+                         "of the tuner's choice")
+    ap.add_argument("--ligand-gemm", type=int, choices=(0, 1), default=None,
+                    help="ligand displacement GEMM off/on instead of the tuner's choice")
     ap.add_argument("--profile-step", action="store_true",
                     help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
                          "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE, "
@@ -297,6 +302,10 @@ def main():
                         wl["steps"], form=sdgso.FP32_GUARD)
         if args.variant is not None:
             run.set_variant(args.variant)
+        if args.xcd_chunk is not None:
+            run.set_work_order(args.xcd_chunk)
+        if args.ligand_gemm is not None:
+            run.set_ligand_gemm(bool(args.ligand_gemm))
         run.steps(1)
         run.sync()
         log("profile step done: %d glowworms, %d in-cutoff pairs in the scoring launch"
@@ -328,6 +337,10 @@ def main():
     else:
         run.set_variant(args.variant)
         chosen, tune_ms = run.variants(), None
+    if args.xcd_chunk is not None:
+        run.set_work_order(args.xcd_chunk)
+    if args.ligand_gemm is not None:
+        run.set_ligand_gemm(bool(args.ligand_gemm))
     # Timed region: args.steps replays of the captured step, with profiling
     # on, so that each replay is the scoring graph and the graph of the rest
     # of the step with HIP events on the run's stream between them (the

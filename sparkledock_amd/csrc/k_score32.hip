@@ -293,11 +293,11 @@ __device__ __forceinline__ void write_partials(const ScoreArgs &a, const WaveCtx
     unsigned int ng = nguard;
     for (int off = 32; off > 0; off >>= 1) ng += __shfl_xor(ng, off, 64);
     if (x.lane == 0) {
-        a.partial[(size_t)x.gid * a.max_lblocks + x.b] = e;
-        a.partial_pairs[(size_t)x.gid * a.max_lblocks + x.b] = np;
+        a.partial[(size_t)x.b * a.n_glow + x.gid] = e;
+        a.partial_pairs[(size_t)x.b * a.n_glow + x.gid] = np;
         if (b2 >= 0) {
-            a.partial[(size_t)x.gid * a.max_lblocks + b2] = 0.0;
-            a.partial_pairs[(size_t)x.gid * a.max_lblocks + b2] = 0;
+            a.partial[(size_t)b2 * a.n_glow + x.gid] = 0.0;
+            a.partial_pairs[(size_t)b2 * a.n_glow + x.gid] = 0;
         }
         if (a.guard_count && ng) atomicAdd(a.guard_count, (unsigned long long)ng);
         if (a.stats) {
@@ -1173,18 +1173,17 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
 }
 
 // Per glowworm: energy = partials in block order (FP64), score, luciferin,
-// reference point (prepare_gso_tasks.cc:513-526, 558, 1571).
+// reference point (prepare_gso_tasks.cc:513-526, 558, 1571).  The partials
+// are block-major, so the threads of a wave read consecutive glowworms.
 __global__ __launch_bounds__(256) void finish32_kernel(ScoreArgs a) {
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     if (gid >= a.n_glow) return;
     const DevComplex &c = a.cx[a.swarm_cx[gid / a.G]];
-    const double *part = a.partial + (size_t)gid * a.max_lblocks;
-    const long long *pp = a.partial_pairs + (size_t)gid * a.max_lblocks;
     double e = 0.0;
     long long np = 0;
     for (int b = 0; b < c.nlblocks; b++) {
-        e += part[b];
-        np += pp[b];
+        e += a.partial[(size_t)b * a.n_glow + gid];
+        np += a.partial_pairs[(size_t)b * a.n_glow + gid];
     }
     const double s = score_of_energy(e);
     a.score[gid] = s;

@@ -141,8 +141,8 @@ struct ScoreArgs {
     // FP32 guard form
     float band;             // guard band (A^2)
     unsigned long long *guard_count;  // FP64 recomputes (optional)
-    double *partial;        // [n_glow][max_lblocks] per-block energies (FP32 form)
-    long long *partial_pairs;  // [n_glow][max_lblocks] per-block pair counts
+    double *partial;        // [max_lblocks][n_glow] per-block energies (FP32 form)
+    long long *partial_pairs;  // [max_lblocks][n_glow] per-block pair counts
     int max_lblocks;
     int nmr;                // receptor modes of every complex of the launch
     const int *work;        // FP32 form: (glowworm, ligand block) pairs, or (glowworm, first
