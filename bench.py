@@ -239,6 +239,22 @@ def traffic_bytes(workload):
     return None
 
 
+def profile_valu_insts(workload):
+    """VALU instructions per scoring launch (SQ_INSTS_VALU) from the committed
+    rocprofv3 counter summary of this workload, if any."""
+    p = os.path.join(ROOT, "profiles", "score32_%s_counters.md" % workload)
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("| SQ_INSTS_VALU |"):
+            return float(line.split("|")[2].split()[0])
+    return None
+
+
+# VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz, one wave64 instruction per 4 cycles
+VALU_ISSUE_PEAK = 256 * 4 * 2.4e9 / 4
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -475,6 +491,17 @@ def main():
     probe_gbs = 4.0 * probe / 1e9
     gather_frac = gather_gbs / probe_gbs
     flop_frac = achieved / FP32_PEAK_TFLOPS
+    # VALU issue, the co-bound: the committed counter profile's instructions
+    # per launch over this run's launch time
+    vi = profile_valu_insts(args.workload)
+    valu_issue = None
+    if vi:
+        valu_issue = {"insts_per_launch": vi, "achieved_per_s": vi / (score_ms * 1e-3),
+                      "peak_per_s": VALU_ISSUE_PEAK,
+                      "frac": vi / (score_ms * 1e-3) / VALU_ISSUE_PEAK,
+                      "source": "SQ_INSTS_VALU of profiles/score32_%s_counters.md over this "
+                                "run's launch time; peak 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles"
+                                % args.workload}
     names = sdgso.tune_variants()
     chosen_names = [names[v] for v in chosen]
     # per complex: in-cutoff pairs of the last scoring and the tuner's launch
@@ -539,6 +566,7 @@ def main():
                                     "a 2.26 MB table, every CU busy",
                      "vs_hbm_peak": {"achieved_GBps": gather_gbs, "peak_GBps": 8000.0,
                                      "frac": gather_gbs / 8000.0},
+                     "valu_issue": valu_issue,
                      "flop": {"achieved_TFLOPs": achieved, "peak_TFLOPs": FP32_PEAK_TFLOPS,
                               "frac": flop_frac,
                               "algorithmic": "8 FP32 flop per in-cutoff pair (difference-form "
