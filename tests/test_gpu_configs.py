@@ -13,7 +13,9 @@ workloads (sparkledock_amd/workloads.py), through the C-ABI.
       kernel's ligand in 68 KB of LDS, above the 64 KB default, and the
       select/move workgroup of 300 x 47-value poses in 130 KB), and the whole FP32 scoring
       launch with a one-block variant, whose 130 M-wave work list is cut at
-      2^26 - 4 waves, checked on the glowworms at the cut.
+      2^26 - 4 waves, checked on the glowworms at the cut, and with the
+      two-block variant in the tuner's c4 configuration (XCD-chunked order,
+      ligand displacement GEMM).
 
 Reference semantics: prepare_gso_tasks.cc:359-822 (the step),
 1522-1615 (scoring), 584-814 (selection and movement); the launch the
@@ -67,17 +69,19 @@ def counts_of(scores):
     return np.rint((4.7 - np.asarray(scores)) / 0.0157).astype(np.int64)
 
 
-def check_energies(inp, poses, got_score, got_pairs, what):
+def check_energies(inp, poses, got_score, got_pairs, what, more=()):
     """Sampled poses of one complex: pair counts equal to the oracle's and
-    energies within the FP32 guard form's bound."""
+    energies within the FP32 guard form's bound; `more` holds further
+    (scores, pair counts) of the same poses to check the same way."""
     s0, p0 = oracle_scores(oracle_complex(inp), poses)
-    assert np.array_equal(got_pairs, p0), what
     sa, _ = oracle_scores(oracle_complex(inp, np.abs(inp["table"])), poses)
     denom = np.maximum((4.7 - sa) / 0.0157, 1.0)
-    e0, e = (4.7 - s0) / 0.0157, (4.7 - np.asarray(got_score)) / 0.0157
-    err = np.abs(e - e0)
-    assert np.all(err <= REL_E * np.abs(e0) + FLOOR_E * denom), \
-        (what, float(np.max(err / np.maximum(np.abs(e0), 1e-300))))
+    e0 = (4.7 - s0) / 0.0157
+    for sc, pc in [(got_score, got_pairs)] + list(more):
+        assert np.array_equal(pc, p0), what
+        err = np.abs((4.7 - np.asarray(sc)) / 0.0157 - e0)
+        assert np.all(err <= REL_E * np.abs(e0) + FLOOR_E * denom), \
+            (what, float(np.max(err / np.maximum(np.abs(e0), 1e-300))))
     return int(p0.sum())
 
 
@@ -289,7 +293,10 @@ def test_c4_whole_launch_across_the_chunk_cut():
     waves, cut into launches at 2^26 - 4 waves (here between two glowworms
     of complex 16).  The glowworms on either side of the cut, and one sampled
     glowworm of every complex, against the oracle; every glowworm's pair
-    count equal to that of the two-block default variant's single launch."""
+    count equal to that of the two-block default variant's single launch, and
+    of that variant in the tuner's c4 configuration (work items in chunks of
+    8192 per XCD, the ligand displacement GEMM on: 100 GB of output), whose
+    sampled glowworms are checked against the oracle too."""
     wl = workloads.build("c4", steps=1)
     G = wl["G"]
     names = sdgso.tune_variants()
@@ -305,9 +312,16 @@ def test_c4_whole_launch_across_the_chunk_cut():
     run.set_variant(names.index("bcast2/ring4/occ4"))
     run.phases(1)
     two = run.read()
+    run.set_work_order(8192)
+    run.set_ligand_gemm(True)
+    assert run.ligand_gemm()[0]
+    run.phases(1)
+    tuned = run.read()
     run.close()
     assert np.array_equal(one["pairs"], two["pairs"])
     assert np.allclose(one["score"], two["score"], rtol=1e-9, atol=1e-9)
+    assert np.array_equal(tuned["pairs"], two["pairs"])
+    assert np.allclose(tuned["score"], two["score"], rtol=1e-9, atol=1e-9)
     # the glowworm whose waves hold the cut, and its neighbours
     cut = int(np.searchsorted(first, CHUNK_WAVES, side="right") - 1)
     say("cut at wave %d: glowworm %d has waves [%d, %d)" % (CHUNK_WAVES, cut, first[cut],
@@ -326,7 +340,8 @@ def test_c4_whole_launch_across_the_chunk_cut():
         idx = np.array(sorted(g for g in sel if a <= g < b))
         if len(idx):
             check_energies(wl["inputs"][k], pos[idx], one["score"][idx], one["pairs"][idx],
-                           "c4 complex %d glowworms %s" % (k, idx.tolist()))
+                           "c4 complex %d glowworms %s" % (k, idx.tolist()),
+                           more=[(tuned["score"][idx], tuned["pairs"][idx])])
 
 
 def test_chunk_cut_inside_a_glowworm():
