@@ -96,6 +96,8 @@ struct DevBuf {
         if (count == 0) return;
         if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
             p = nullptr;
+            // clear the runtime's last error, which a later launch check would report
+            (void)hipGetLastError();
             fail(SDGSO_ENOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) + " bytes failed");
         }
         n = count;
@@ -1332,7 +1334,6 @@ int sdgso_run_tune(sdgso_run r, int repeats, double *times_ms, int *chosen) {
             } catch (const Failure &f) {
                 // its output does not fit in device memory: it stays off
                 if (f.code != SDGSO_ENOMEM) throw;
-                (void)hipGetLastError();  // the failed allocation's error
                 set_lgemm(r, false);
                 can = false;
             }

@@ -197,3 +197,35 @@ def test_tuner_keeps_the_faster_form():
     p = run.read()["pairs"]
     assert np.array_equal(p, pr.oracle_scores(c["pos"].reshape(-1, pr.P))[2])
     run.close()
+
+
+def test_gemm_that_does_not_fit_is_refused_and_leaves_no_error():
+    """A run whose ligand displacement output would exceed the GPU's memory
+    (4 M glowworms x 100 ligand blocks x 768 B = 307 GB): switching the GEMM
+    on fails with SDGSO_ENOMEM and leaves it off; the run still scores, and
+    a later run on the same thread launches without a stale runtime error."""
+    rng = np.random.default_rng(93)
+    nr, nl, S, G = 64, 6400, 4000, 1000
+    r = rng.normal(size=(nr, 3)) * 4.0
+    lg = rng.normal(size=(nl, 3)) * 12.0
+    table = (rng.normal(size=168 * 168 * 20) * 0.5).astype(np.float32)
+    lmodes = rng.normal(size=(1, nl, 3)) * 0.01
+    cx = sdgso.Complex(r, rng.integers(0, 168, nr).astype(np.int32), lg,
+                       rng.integers(0, 168, nl).astype(np.int32), table, np.zeros(3),
+                       None, lmodes)
+    pose = np.array([30.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0, 0.5])
+    run = sdgso.Run(cx, S, G, np.tile(pose, S * G), None, 0, form=sdgso.FP32_GUARD)
+    with pytest.raises(sdgso.SdgsoError) as e:
+        run.set_ligand_gemm(True)
+    assert e.value.code == 3
+    assert run.ligand_gemm() == (False, 0)
+    run.close()
+    cx.close()
+    c = GC.case(94, 300, 150, 2, 3, 2, 8)
+    pr = GC.Pair(c)
+    poses = c["pos"].reshape(-1, pr.P)
+    small = sdgso.Run(pr.gpu, 16, 1, poses, None, 0, form=sdgso.FP32_GUARD)
+    small.set_ligand_gemm(True)
+    small.phases(1)
+    assert np.array_equal(small.read()["pairs"], pr.oracle_scores(poses)[2])
+    small.close()
