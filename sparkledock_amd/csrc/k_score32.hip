@@ -2,11 +2,14 @@
 //
 // Work unit: one wavefront per (glowworm, ligand block of 64 atoms), or per
 // (glowworm, pair of neighbouring blocks) in the two-block variants, taken
-// from a work list that keeps the blocks of one glowworm adjacent.  Lane l
-// owns ligand atom lperm[block*64 + l]: its FP64 coordinates plus an FP32
-// mode sum, rotated by the pose's FP64 matrix and translated
-// (prepare_gso_tasks.cc:487-526 up to rounding), then rounded once to a
-// 2^-20 A fixed-point grid whose origin is the (first) block's centre.
+// from a work list that keeps the blocks of one glowworm adjacent, in list
+// order or in chunks per XCD (ScoreArgs::xcd_chunk).  Lane l owns ligand
+// atom lperm[block*64 + l]: its FP64 coordinates plus its FP32 mode
+// displacement (summed in the set-up, or read from the run's ligand
+// displacement GEMM, ScoreArgs::ldisp), rotated by the pose's FP64 matrix
+// and translated (prepare_gso_tasks.cc:487-526 up to rounding), then rounded
+// once to a 2^-20 A fixed-point grid whose origin is the (first) block's
+// centre.
 //
 // Receptor tiles: compact groups of 64 atoms (recursive bisection) with
 // bounding spheres.  The wave tests 64 tile spheres per instruction against
