@@ -374,7 +374,12 @@ int main(int argc, char **argv) {
         // poses a few times; the run's state is unchanged)
         int chosen = 0;
         check(sdgso_run_tune(run, 1, nullptr, &chosen), "sdgso_run_tune");
-        std::printf("scoring kernel: %s\n", sdgso_tune_variant_name(chosen));
+        int chunk = 0, lgemm = 0;
+        check(sdgso_run_work_order(run, &chunk, nullptr), "sdgso_run_work_order");
+        check(sdgso_run_ligand_gemm(run, &lgemm, nullptr, nullptr), "sdgso_run_ligand_gemm");
+        std::printf("scoring kernel: %s, work items %s, ligand displacement %s\n",
+                    sdgso_tune_variant_name(chosen), chunk ? "in chunks per XCD" : "in list order",
+                    lgemm ? "by the matrix-core GEMM" : "in the scoring set-up");
     }
     std::printf("prepare docking\n");
     const auto t0 = std::chrono::steady_clock::now();
