@@ -387,6 +387,47 @@ __device__ __forceinline__ float dpp_f32(float v) {
         float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), kCtrl, kRowMask, 0xF, false));
 }
 
+// Scans within rows of 16 lanes, each step v += v' * g with v' the value of
+// the lane s below in the same row, as one v_fmac_f32_dpp per step (from
+// dpp_f32 and an fma the compiler builds a v_mov_b32 of zero, a
+// v_mov_b32_dpp and a v_fmac_f32).  A lane whose source lies outside its
+// row keeps v, as adding 0 * g would.  Each s_nop gives the two wait states
+// a DPP read needs after a VALU write of its source.
+//
+// acc + x(s = 1) gb, then steps s = 1 (g1) and s = 2 (g2)
+__device__ __forceinline__ float dpp_scan_x12(float acc, float x, float gb, float g1, float g2) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %1, %2 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %3 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %4 row_shr:2 row_mask:0xf bank_mask:0xf"
+        : "+v"(acc)
+        : "v"(x), "v"(gb), "v"(g1), "v"(g2));
+    return acc;
+}
+
+// steps s = 1 (g1), 2 (g2) and 4 (g4)
+__device__ __forceinline__ float dpp_scan_124(float v, float g1, float g2, float g4) {
+    asm("s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %1 row_shr:1 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %2 row_shr:2 row_mask:0xf bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_fmac_f32_dpp %0, %0, %3 row_shr:4 row_mask:0xf bank_mask:0xf"
+        : "+v"(v)
+        : "v"(g1), "v"(g2), "v"(g4));
+    return v;
+}
+
+// float4 at byte offset off from p (a 32-bit offset on the scalar base).
+__device__ __forceinline__ float4 gld_at(const float4 *p, unsigned off) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const f4 v = *(const __attribute__((address_space(1))) f4 *)(
+        (const __attribute__((address_space(1))) char *)p + off);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+
 // Sum of v over aligned groups of kC lanes (8, 16 or 32), exact in the
 // group's last lane: quad swaps, then row_shr:4 (8-lane sums in lanes 4-7 and
 // 12-15 of a row), row_shr:8 (16-lane sums in lanes 12-15) and row_bcast:15
@@ -435,7 +476,9 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
     constexpr int kPend = 64 * kTilesB + 64;  // pending near slots: a batch's worth + one round
     constexpr int kQueue = queue_len<kUnroll, kMfma>();
     constexpr int kGroup = 64 * kUnroll;  // pairs of one group of rounds
-    // a pending entry: receptor slot (low 16 bits) and, with two blocks, near flags
+    // a pending entry: receptor slot (low 16 bits) and, with two blocks, near
+    // flags (bits 30-31, which the 32-bit product of the entry with a mode
+    // record's size in bytes drops, so that product is the record's offset)
     using PdT = std::conditional_t<kLB == 2, unsigned, unsigned short>;
     __shared__ int4 s_rt[kWpb][kRtRows];     // packed near receptor atoms: fixed point + row
     __shared__ float4 s_rf[kWpb][64];        //   this batch's atoms in A: x, y, z, |r|^2
@@ -729,7 +772,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         if (lane < n) {
             const unsigned e = pd[lane];
             slot = (int)(e & 0xffffu);
-            fl = e >> 16;
+            fl = e >> 30;
             rxy = gld2(rrel4, 2 * (size_t)slot);
             rzw = gld2(rrel4, 2 * (size_t)slot + 1);
         }
@@ -749,6 +792,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             // lane adding only from lanes of its own coordinate) leaves the
             // coordinate's sum in its last lane, which stages it in rf
             const int j = lane & (kC - 1);
+            const unsigned j16 = 16u * (unsigned)j;  // byte offset of float4 j
             const int q = (nmr + 3) >> 2;
             const int comp = (j >= q ? 1 : 0) + (j >= 2 * q ? 1 : 0) + (j >= 3 * q ? 1 : 0);
             const int pos = j - comp * q;
@@ -770,7 +814,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 for (int r = 0; r < kDispRounds; r++) {
                     av[r] = (k0 + r) * kApr + lane / kC;
                     qv[r] = (k0 + r < nround && av[r] < n)
-                                ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * kC + j)
+                                ? gld_at(rmodA, (unsigned)pd[av[r]] * (16u * kC) + j16)
                                 : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
 #pragma unroll
@@ -780,9 +824,8 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                     v = __builtin_fmaf(qv[r].y, w4[1], v);
                     v = __builtin_fmaf(qv[r].z, w4[2], v);
                     v = __builtin_fmaf(qv[r].w, w4[3], v);
-                    v = __builtin_fmaf(dpp_f32<0x111>(v), g1, v);  // row_shr:1
-                    if (q > 1) v = __builtin_fmaf(dpp_f32<0x112>(v), g2, v);  // row_shr:2
-                    if (q > 4) v = __builtin_fmaf(dpp_f32<0x114>(v), g4, v);  // row_shr:4
+                    // q >= 3 here (11 to 20 modes); with q <= 4 every g4 is 0
+                    v = dpp_scan_124(v, g1, g2, g4);
                     if (last && av[r] < n) ((float *)&rf[av[r]])[comp] = v;
                 }
             }
@@ -798,6 +841,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             // lanes of one first coordinate (at most three: shifts 1 and 2)
             // leaves each coordinate's sum in its last lane
             const int j = lane & 7;
+            const unsigned j16 = 16u * (unsigned)j;  // byte offset of float4 j
             const int cs = nmr > 4 ? nmr : 4;
             const int ka = (4 * j) / cs;
             float wa[4], wb[4];
@@ -821,7 +865,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 for (int r = 0; r < kDispRounds; r++) {
                     av[r] = (k0 + r) * kApr + lane / 8;
                     qv[r] = (k0 + r < nround && av[r] < n)
-                                ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * 8 + j)
+                                ? gld_at(rmodA, (unsigned)pd[av[r]] * (16u * 8u) + j16)
                                 : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
 #pragma unroll
@@ -834,9 +878,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                     float vb = qv[r].y * wb[1];
                     vb = __builtin_fmaf(qv[r].z, wb[2], vb);
                     vb = __builtin_fmaf(qv[r].w, wb[3], vb);
-                    float v = __builtin_fmaf(dpp_f32<0x111>(vb), gb, va);  // row_shr:1
-                    v = __builtin_fmaf(dpp_f32<0x111>(v), g1, v);
-                    v = __builtin_fmaf(dpp_f32<0x112>(v), g2, v);          // row_shr:2
+                    const float v = dpp_scan_x12(va, vb, gb, g1, g2);
                     if (last && av[r] < n) ((float *)&rf[av[r]])[ka] = v;
                 }
             }
@@ -844,6 +886,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
             if (lane < n) D = rf[lane];
         } else if (nmr > 0) {
             const int j = lane & (kC - 1);
+            const unsigned j16 = 16u * (unsigned)j;  // byte offset of float4 j
             float wx[4], wy[4], wz[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) {
@@ -862,7 +905,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
                 for (int r = 0; r < kDispRounds; r++) {
                     av[r] = (k0 + r) * kApr + lane / kC;
                     q[r] = (k0 + r < nround && av[r] < n)
-                               ? gld(rmodA, (size_t)(pd[av[r]] & 0xffffu) * kC + j)
+                               ? gld_at(rmodA, (unsigned)pd[av[r]] * (16u * kC) + j16)
                                : make_float4(0.f, 0.f, 0.f, 0.f);
                 }
 #pragma unroll
@@ -1099,7 +1142,7 @@ __global__ __launch_bounds__(64 * kWpb, kMinWaves) void score32b_kernel(ScoreArg
         if (nearb)
             pd[npd + __builtin_amdgcn_mbcnt_hi((unsigned)(m >> 32),
                                                __builtin_amdgcn_mbcnt_lo((unsigned)m, 0u))] =
-                (PdT)((unsigned)(t * kTile + lane) | (fl << 16));
+                (PdT)((unsigned)(t * kTile + lane) | (fl << 30));
         npd += __popcll(m);
         n_near += __popcll(m);
     };
