@@ -53,16 +53,34 @@ __global__ __launch_bounds__(64) void ligand_disp_kernel(LigDispArgs a) {
     for (int o = 0; o < 6; o++)
 #pragma unroll
         for (int v = 0; v < 16; v++) acc[o][v] = 0.f;
-    for (int m0 = 0; m0 < nm; m0 += 2) {
-        const int m = m0 + kk;
-        const float av = (g < gend && m < nm) ? (float)gld(e, m) : 0.f;
+    // Every load is unconditional (a mode index past nm is clamped to the last
+    // mode, whose finite values then meet an A of zero), so the loads of two
+    // steps (four modes) are issued together: one round trip per two steps
+    // instead of one per MFMA.
+    for (int m0 = 0; m0 < nm; m0 += 4) {
+        int mc[2];
+        double ev[2];
+        float bv[2][6];
 #pragma unroll
-        for (int o = 0; o < 6; o++) {
-            // B operand of lane l: mode m0 + l / 32, column q = 32 o + l % 32
-            // (coordinate q / 64 of slot q % 64; M row 3 m + coordinate)
-            const int q = 32 * o + row;
-            const float bv = m < nm ? gld(M, (size_t)(3 * m + (q >> 6)) * kBlock + (q & 63)) : 0.f;
-            acc[o] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv, acc[o], 0, 0, 0);
+        for (int h = 0; h < 2; h++) {
+            const int m = m0 + 2 * h + kk;
+            mc[h] = m < nm ? m : nm - 1;
+            ev[h] = gld(e, mc[h]);
+#pragma unroll
+            for (int o = 0; o < 6; o++) {
+                // B operand of lane l: mode m0 + 2h + l / 32, column q = 32 o + l % 32
+                // (coordinate q / 64 of slot q % 64; M row 3 m + coordinate)
+                const int q = 32 * o + row;
+                bv[h][o] = gld(M, (size_t)(3 * mc[h] + (q >> 6)) * kBlock + (q & 63));
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int m = m0 + 2 * h + kk;
+            const float av = (g < gend && m < nm) ? (float)ev[h] : 0.f;
+#pragma unroll
+            for (int o = 0; o < 6; o++)
+                acc[o] = __builtin_amdgcn_mfma_f32_32x32x2f32(av, bv[h][o], acc[o], 0, 0, 0);
         }
     }
     // accumulator v of lane l: row 8 (v / 4) + 4 (l / 32) + v % 4, column l % 32
