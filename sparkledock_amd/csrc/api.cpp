@@ -222,6 +222,7 @@ struct sdgso_run_s {
     DevBuf<DevComplex> dcx;
     DevBuf<int> swarm_cx;
     DevBuf<double> pos, luciferin, score, vision, ref, stream, partial;
+    DevBuf<double> mp;  // select/move: the chosen neighbours' pre-move poses, [P][N]
     DevBuf<int> nnei, moved, select, step, work, work2;
     DevBuf<int64_t> pairs;
     DevBuf<long long> partial_pairs;
@@ -752,6 +753,7 @@ static void enqueue_select_move(sdgso_run r, int do_select, int do_move) {
     m.step_nm = r->params.step_nmodes;
     m.do_select = do_select;
     m.do_move = do_move;
+    m.mp = r->mp.p;
     HIPCHK(launch_select_move(m, r->stream_h));
 }
 
@@ -914,13 +916,12 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
         r->n_steps = n_steps;
         // LDS limits of the kernels
         const size_t lds64 = sizeof(double) * (3 * (size_t)r->max_nl + 512) + sizeof(int) * r->max_nl;
-        const size_t ldsg = sizeof(double) * (4 * (size_t)r->G + (size_t)((r->G + 63) / 64 * 64) * r->P);
         if (form == SDGSO_FP64 && lds64 > 160 * 1024)
             fail(SDGSO_EINVAL, "ligand too large for the FP64 form (" + std::to_string(r->max_nl) + " atoms)");
-        if (ldsg > 160 * 1024) fail(SDGSO_EINVAL, "glowworms x pose length exceed the LDS of one workgroup");
         r->dcx.upload(descs);
         r->swarm_cx.upload(scx);
         r->pos.upload(pos, N * r->P);
+        r->mp.alloc(N * r->P);
         std::vector<double> init((size_t)N);
         std::fill(init.begin(), init.end(), r->params.luciferin0);
         r->luciferin.upload(init);

@@ -3,12 +3,13 @@
 // cal_move_neighbors + move_position_v2, prepare_gso_task.cu:791-1136).
 //
 // LDS holds the swarm's luciferin and reference points (broadcast reads in
-// the O(G) neighbour scan) and each glowworm's copy of its chosen neighbour's
-// pre-move pose.  The neighbour list is never stored: pass 1 counts the
-// neighbours and sums the luciferin differences in index order, pass 2 walks
-// the roulette with the same per-neighbour probabilities
-// (prepare_gso_tasks.cc:1424-1488).  After a barrier every lane updates its
-// own pose in place.
+// the O(G) neighbour scan), 32 B per glowworm, so several swarms share a CU.
+// The neighbour list is never stored: pass 1 counts the neighbours and sums
+// the luciferin differences in index order, pass 2 walks the roulette with
+// the same per-neighbour probabilities (prepare_gso_tasks.cc:1424-1488).
+// Each lane then copies its chosen neighbour's pre-move pose to the run's
+// [P][N] array a.mp (consecutive lanes, consecutive addresses), and after a
+// barrier every lane updates its own pose in place from that copy.
 #include "dev_common.h"
 
 namespace sd {
@@ -49,17 +50,19 @@ __device__ void slerp64(double *self, double *other, double t) {
     }
 }
 
-__device__ void move_extent64(double *self, const double *other, int n, double step) {
+// other's value i at other[i * stride]
+__device__ void move_extent64(double *self, const double *other, size_t stride, int n,
+                              double step) {
     // prepare_gso_tasks.cc:1352-1368
     double nn = 0;
     for (int i = 0; i < n; i++) {
-        const double d = other[i] - self[i];
+        const double d = other[i * stride] - self[i];
         nn += d * d;
     }
     nn = sqrt(nn);
     if (nn >= 1e-8) {
         for (int i = 0; i < n; i++) {
-            double d = other[i] - self[i];
+            double d = other[i * stride] - self[i];
             d *= step / nn;
             self[i] += d;
         }
@@ -73,8 +76,8 @@ __global__ __launch_bounds__(1024) void select_move_kernel(SelectMoveArgs a) {
     const int G = a.G, P = a.P;
     double *lu = lds;               // [G]
     double *rf = lu + G;            // [G][3]
-    double *mp_all = rf + 3 * G;    // [G][P] copy of the chosen neighbour's pose
     const size_t base = (size_t)s * G;
+    const size_t N = (size_t)a.n_swarms * G;
     if (g < G) {
         lu[g] = a.luciferin[base + g];
         rf[3 * g + 0] = a.ref[(base + g) * 3 + 0];
@@ -143,14 +146,17 @@ __global__ __launch_bounds__(1024) void select_move_kernel(SelectMoveArgs a) {
         }
     }
     if (!a.do_move) return;
-    // copy the chosen neighbour's pre-move pose; nobody writes pos before the barrier
-    double *mp = mp_all + (size_t)g * P;
+    // copy the chosen neighbour's pre-move pose; nobody writes pos before the
+    // barrier, and each lane reads back only its own copy
     if (g < G) {
         const double *src = a.pos + (base + sel) * P;
-        for (int p = 0; p < P; p++) mp[p] = src[p];
+        for (int p = 0; p < P; p++) a.mp[(size_t)p * N + base + g] = src[p];
     }
     __syncthreads();
     if (g >= G) return;
+    double mp[7];
+    for (int p = 0; p < 7; p++) mp[p] = a.mp[(size_t)p * N + base + g];
+    const double *mpx = a.mp + 7 * N + base + g;  // extent i at mpx[i * N]
     double *cur = a.pos + (base + g) * P;
     const int moved = (sel != g);
     a.moved[base + g] = moved;
@@ -176,8 +182,8 @@ __global__ __launch_bounds__(1024) void select_move_kernel(SelectMoveArgs a) {
         }
         slerp64(q, o, a.step_r);
         for (int i = 0; i < 4; i++) cur[3 + i] = q[i];
-        move_extent64(cur + 7, mp + 7, a.nmr, a.step_nm);
-        move_extent64(cur + 7 + a.nmr, mp + 7 + a.nmr, a.nml, a.step_nm);
+        move_extent64(cur + 7, mpx, N, a.nmr, a.step_nm);
+        move_extent64(cur + 7 + a.nmr, mpx + (size_t)a.nmr * N, N, a.nml, a.step_nm);
     }
     // prepare_gso_tasks.cc:806-808
     double v = a.vision[base + g] + a.beta * (a.max_neighbors - nn);
@@ -188,13 +194,9 @@ __global__ __launch_bounds__(1024) void select_move_kernel(SelectMoveArgs a) {
 
 hipError_t launch_select_move(const SelectMoveArgs &a, hipStream_t s) {
     if (a.n_swarms <= 0) return hipSuccess;
+    if (a.G < 1 || a.G > 1024 || (a.do_move && !a.mp)) return hipErrorInvalidValue;
     const int threads = ((a.G + 63) / 64) * 64;
-    const size_t lds = sizeof(double) * ((size_t)4 * a.G + (size_t)threads * a.P);
-    if (lds > 64 * 1024) {
-        const hipError_t e = hipFuncSetAttribute((const void *)select_move_kernel,
-                                                 hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        if (e != hipSuccess) return e;
-    }
+    const size_t lds = sizeof(double) * 4 * (size_t)a.G;  // at most 32 KB
     hipLaunchKernelGGL(select_move_kernel, dim3(a.n_swarms), dim3(threads), lds, s, a);
     return hipGetLastError();
 }

@@ -201,6 +201,7 @@ struct SelectMoveArgs {
     int max_neighbors;
     double step_t, step_r, step_nm;
     int do_select, do_move;
+    double *mp;             // [P][N] each glowworm's copy of its chosen neighbour's pre-move pose
 };
 
 // Final coordinates of chosen atoms for a range of glowworms of one complex.
