@@ -915,7 +915,7 @@ int sdgso_run_create(int n_complex, const sdgso_complex *complexes, const int *n
         r->N = (int)N;
         r->n_steps = n_steps;
         // LDS limits of the kernels
-        const size_t lds64 = sizeof(double) * (3 * (size_t)r->max_nl + 512) + sizeof(int) * r->max_nl;
+        const size_t lds64 = score64_lds_bytes(r->max_nl);
         if (form == SDGSO_FP64 && lds64 > 160 * 1024)
             fail(SDGSO_EINVAL, "ligand too large for the FP64 form (" + std::to_string(r->max_nl) + " atoms)");
         r->dcx.upload(descs);

@@ -214,6 +214,21 @@ __device__ __forceinline__ void ref_point64(const DevComplex &c, const double *p
     out[2] = z + pose[2];
 }
 
+// Bin argument d of one pair in FP64, (int)(sqrt(d2) * 2 - 1)
+// (prepare_gso_tasks.cc:1542-1556); false when the pair is outside (0, 225].
+__device__ __forceinline__ bool pair_bin64(double rx, double ry, double rz, double lx, double ly,
+                                           double lz, int &d) {
+    const double dx = rx - lx;
+    const double dy = ry - ly;
+    const double dz = rz - lz;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    if (d2 <= kCutoff2 && d2 > 0.0) {
+        d = (int)(sqrt(d2) * 2.0 - 1.0);
+        return true;
+    }
+    return false;
+}
+
 // DFIRE contribution of one pair in FP64 (prepare_gso_tasks.cc:1542-1566);
 // returns false when the pair is outside (0, 225].  The table is the FP64
 // one (FP64 form) or its FP32 copy (the FP32 form's guard-band pairs, whose
@@ -222,12 +237,8 @@ template <class TableT>
 __device__ __forceinline__ bool pair_term64(double rx, double ry, double rz, double lx, double ly,
                                             double lz, int ra, int lb, const TableT *table,
                                             double &val) {
-    const double dx = rx - lx;
-    const double dy = ry - ly;
-    const double dz = rz - lz;
-    const double d2 = dx * dx + dy * dy + dz * dz;
-    if (d2 <= kCutoff2 && d2 > 0.0) {
-        const int d = (int)(sqrt(d2) * 2.0 - 1.0);
+    int d;
+    if (pair_bin64(rx, ry, rz, lx, ly, lz, d)) {
         val = (double)gld(table, ra + lb + bin_of_d(d));
         return true;
     }

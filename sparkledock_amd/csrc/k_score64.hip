@@ -3,11 +3,19 @@
 // is the calculate_dfire kernel of source/dock/src/cuda/gpu_baseline:492).
 //
 // Layout per workgroup:
-//   LDS  ligand pose in FP64 SoA (x[nl], y[nl], z[nl]) + type*20 [nl]
-//        + two 256-entry row-sum buffers
+//   LDS  ligand pose in FP64 SoA (x[nlp], y[nlp], z[nlp]) + type*20 [nlp],
+//        nl padded to nlp = a multiple of 16 with far-away atoms, + two
+//        256-entry row-sum buffers
 //   lane one receptor row i at a time (i = chunk*256 + tid), its FP64
-//        coordinates displaced in registers; ligand atoms are LDS broadcasts
+//        coordinates displaced in registers; ligand atoms are LDS broadcasts,
+//        two atoms' coordinates per 16-byte read
 //
+// A pair's table value is a buffer load whose offset lies past the table
+// (the load returns +0.0) unless the pair is inside (0, 225]; the loads go
+// into a ring of eight and each is added to the row sum eight pairs later, so
+// a gather's latency is not waited for at its pair.  The additions stay in
+// pair order, and adding +0.0 leaves an FP64 sum unchanged (a row sum that
+// starts at +0.0 is never -0.0).
 // The energy is summed exactly as prepare_gso_tasks.cc:1539-1570 does: each
 // row sum v_i runs over j in order (one lane), and thread 0 adds the v_i in
 // row order (double-buffered, one barrier per chunk), so the FP64 energy is
@@ -17,32 +25,35 @@
 
 namespace sd {
 
-constexpr int kScore64Threads = 256;
-
 __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
-    extern __shared__ double lds[];
+    extern __shared__ __align__(16) double lds[];
     const int gid = blockIdx.x;
     if (gid >= a.n_glow) return;
     const int tid = threadIdx.x;
     const int swarm = gid / a.G;
     const DevComplex &c = a.cx[a.swarm_cx[swarm]];
     const int nl = c.nl, nr = c.nr;
+    const int nlp = score64_padded_nl(nl);
     double *lx = lds;
-    double *ly = lx + nl;
-    double *lz = ly + nl;
-    double *vbuf = lz + nl;                       // [2][256]
-    int *lt = (int *)(vbuf + 2 * kScore64Threads);  // [nl]
+    double *ly = lx + nlp;
+    double *lz = ly + nlp;
+    double *vbuf = lz + nlp;                        // [2][256]
+    int *lt = (int *)(vbuf + 2 * kScore64Threads);  // [nlp]
 
     const double *pose = a.pos + (size_t)gid * a.P;
     double inv[4];
     quat_inverse(pose + 3, inv);
-    for (int j = tid; j < nl; j += kScore64Threads) {
-        double x, y, z;
-        ligand_atom64(c, pose, inv, j, x, y, z);
+    for (int j = tid; j < nlp; j += kScore64Threads) {
+        double x = 1e30, y = 1e30, z = 1e30;  // padding: no pair within the cutoff
+        int t = 0;
+        if (j < nl) {
+            ligand_atom64(c, pose, inv, j, x, y, z);
+            t = gld(c.ltype, j);
+        }
         lx[j] = x;
         ly[j] = y;
         lz[j] = z;
-        lt[j] = gld(c.ltype, j);
+        lt[j] = t;
     }
     if (tid == 0) {
         double r[3];
@@ -54,25 +65,93 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     __syncthreads();
 
     const double *ext = pose + 7;
-    const double *table = c.table64;
+    const __amdgpu_buffer_rsrc_t tbl = __builtin_amdgcn_make_buffer_rsrc(
+        (void *)c.table64, (short)0, (SDGSO_TABLE_LEN + 20) * 8, 0x00020000);
+    constexpr unsigned kNoLoad64 = 0x80000000u;  // past the table: the load returns +0.0
     double energy = 0.0;  // thread 0 only
     long long npairs = 0;
     const int nchunks = (nr + kScore64Threads - 1) / kScore64Threads;
     for (int ch = 0; ch < nchunks; ch++) {
         const int i = ch * kScore64Threads + tid;
-        double v = 0.0;
+        // rows past nr sit far away and find no pair, so every lane runs the loop
+        double rx = -1e30, ry = -1e30, rz = -1e30;
+        int ra = 0;
         if (i < nr) {
-            double rx, ry, rz;
             receptor_atom64(c, ext, i, rx, ry, rz);
-            const int ra = gld(c.rtype, i);
-            for (int j = 0; j < nl; j++) {
-                double val;
-                if (pair_term64(rx, ry, rz, lx[j], ly[j], lz[j], ra, lt[j], table, val)) {
-                    v += val;
-                    npairs++;
+            ra = gld(c.rtype, i);
+        }
+        double v = 0.0;
+        double ring[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) ring[k] = 0.0;
+        // the offsets of eight pairs (no memory access under a branch)
+        auto offsets = [&](int j0, unsigned(&off)[8]) {
+#pragma unroll
+            for (int h = 0; h < 2; h++) {
+                const int jb = j0 + 4 * h;
+                const double2 x01 = *(const double2 *)(lx + jb), x23 = *(const double2 *)(lx + jb + 2);
+                const double2 y01 = *(const double2 *)(ly + jb), y23 = *(const double2 *)(ly + jb + 2);
+                const double2 z01 = *(const double2 *)(lz + jb), z23 = *(const double2 *)(lz + jb + 2);
+                const int4 t4 = *(const int4 *)(lt + jb);
+                const double xs[4] = {x01.x, x01.y, x23.x, x23.y};
+                const double ys[4] = {y01.x, y01.y, y23.x, y23.y};
+                const double zs[4] = {z01.x, z01.y, z23.x, z23.y};
+                int ts[4] = {t4.x, t4.y, t4.z, t4.w};
+                // the types are read here, not under the branches below
+                asm volatile("" : "+v"(ts[0]), "+v"(ts[1]), "+v"(ts[2]), "+v"(ts[3]));
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    // pair_bin64 with the test d2 > 0 (coincident atoms, rare)
+                    // under the cutoff branch: one compare less per pair
+                    const int row = ra + ts[k];
+                    const double dx = rx - xs[k];
+                    const double dy = ry - ys[k];
+                    const double dz = rz - zs[k];
+                    const double d2 = dx * dx + dy * dy + dz * dz;
+                    unsigned o = kNoLoad64;
+                    if (d2 <= kCutoff2) {
+                        const int d = (int)(sqrt(d2) * 2.0 - 1.0);
+                        const bool in = d2 > 0.0;
+                        o = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
+                        npairs += in ? 1 : 0;
+                    }
+                    off[4 * h + k] = o;
                 }
             }
+        };
+        // In pair order, the values of the previous eight pairs into the row
+        // sum, and these pairs' loads into the freed slots.  The empty asm
+        // statements order them: the adds after this group's arithmetic (a
+        // slot's load then has a whole group behind it), the loads after the
+        // adds (a load reuses its slot's registers).  `keep`, the previous
+        // loads' offsets, stays untouched until here: the compiler makes a
+        // register that a load in flight reads wait for that load before
+        // anything else is written into it.
+        auto issue = [&](const unsigned(&off)[8], const unsigned(&keep)[8]) {
+#pragma unroll
+            for (int k = 0; k < 8; k++) asm volatile("" : "+v"(ring[k]) : "v"(off[7]));
+#pragma unroll
+            for (int k = 0; k < 8; k++) v += ring[k];
+            asm volatile("" ::"v"(keep[0]), "v"(keep[1]), "v"(keep[2]), "v"(keep[3]),
+                         "v"(keep[4]), "v"(keep[5]), "v"(keep[6]), "v"(keep[7]));
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                unsigned o = off[k];
+                asm volatile("" : "+v"(o) : "v"(v));
+                ring[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(tbl, o, 0, 0));
+            }
+        };
+        unsigned offA[8], offB[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) offB[k] = kNoLoad64;
+        for (int j0 = 0; j0 < nlp; j0 += 16) {
+            offsets(j0, offA);
+            issue(offA, offB);
+            offsets(j0 + 8, offB);
+            issue(offB, offA);
         }
+#pragma unroll
+        for (int k = 0; k < 8; k++) v += ring[k];
         double *vb = vbuf + (ch & 1) * kScore64Threads;
         vb[tid] = v;
         __syncthreads();
@@ -103,8 +182,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
 
 hipError_t launch_score64(const ScoreArgs &a, int max_nl, hipStream_t s) {
     if (a.n_glow <= 0) return hipSuccess;
-    const size_t lds = sizeof(double) * (3 * (size_t)max_nl + 2 * kScore64Threads) +
-                       sizeof(int) * (size_t)max_nl;
+    const size_t lds = score64_lds_bytes(max_nl);
     if (lds > 64 * 1024) {
         const hipError_t e = hipFuncSetAttribute((const void *)score64_kernel,
                                                  hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);

@@ -60,6 +60,15 @@ constexpr double kQuantScale = 1048576.0;  // 2^20: fixed-point units per A
 constexpr double kCutoff2 = 225.0;
 constexpr int kMaxTiles32 = 1024;      // FP32 form: receptor tiles per complex (65536 atoms)
 constexpr int kStatSlots = 256;        // slots per filter-statistics counter
+constexpr int kScore64Threads = 256;   // FP64 form: threads per workgroup (one glowworm)
+
+// FP64 form: ligand atoms padded to a multiple of 16 (far-away atoms), and its
+// LDS: FP64 x, y, z and an int type per padded atom, two row-sum buffers.
+inline __host__ __device__ int score64_padded_nl(int nl) { return (nl + 15) & ~15; }
+inline size_t score64_lds_bytes(int max_nl) {
+    const size_t nlp = (size_t)score64_padded_nl(max_nl);
+    return sizeof(double) * (3 * nlp + 2 * kScore64Threads) + sizeof(int) * nlp;
+}
 
 // float4 per receptor slot that hold its FP32 modes (rmodA): 3*nm floats in
 // whole 128-byte lines, rounded up to 8, 16 or 32 float4 (nm <= 32).
