@@ -87,8 +87,7 @@ extern "C" int sdgso_score_poses_noalloc(sdgso_complex c, int n_poses, const dou
     }
     if (n_poses == 0) return SDGSO_OK;
     // the limits sdgso_run_create applies to the same complex
-    if (form == SDGSO_FP64 &&
-        sizeof(double) * (3 * (size_t)v.nl + 512) + sizeof(int) * (size_t)v.nl > 160 * 1024) {
+    if (form == SDGSO_FP64 && score64_lds_bytes(v.nl) > 160 * 1024) {
         set_error("ligand too large for the FP64 form (" + std::to_string(v.nl) + " atoms)");
         return SDGSO_EINVAL;
     }
