@@ -123,11 +123,11 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         // sum, and these pairs' loads into the freed slots.  The empty asm
         // statements order them: the adds after this group's arithmetic (a
         // slot's load then has a whole group behind it), the loads after the
-        // adds (a load reuses its slot's registers).  `keep`, the previous
-        // loads' offsets, stays untouched until here: the compiler makes a
-        // register that a load in flight reads wait for that load before
-        // anything else is written into it.
-        auto issue = [&](const unsigned(&off)[8], const unsigned(&keep)[8]) {
+        // adds (a load reuses its slot's registers).  `keep`, the registers
+        // that the previous loads read their offsets from, stays untouched
+        // until here: the compiler makes an instruction that writes a
+        // register which a load in flight reads wait for that load.
+        auto issue = [&](const unsigned(&off)[8], const unsigned(&keep)[8], unsigned(&used)[8]) {
 #pragma unroll
             for (int k = 0; k < 8; k++) asm volatile("" : "+v"(ring[k]) : "v"(off[7]));
 #pragma unroll
@@ -139,16 +139,20 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
                 unsigned o = off[k];
                 asm volatile("" : "+v"(o) : "v"(v));
                 ring[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(tbl, o, 0, 0));
+                used[k] = o;  // the register the load reads
             }
         };
-        unsigned offA[8], offB[8];
+        unsigned offA[8], offB[8], useA[8], useB[8];
 #pragma unroll
-        for (int k = 0; k < 8; k++) offB[k] = kNoLoad64;
+        for (int k = 0; k < 8; k++) useB[k] = kNoLoad64;
+        // nothing in flight on entry (the row's loads done), so that inside
+        // the loop only the ring's loads are waited for
+        __builtin_amdgcn_s_waitcnt(0);
         for (int j0 = 0; j0 < nlp; j0 += 16) {
             offsets(j0, offA);
-            issue(offA, offB);
+            issue(offA, useB, useA);
             offsets(j0 + 8, offB);
-            issue(offB, offA);
+            issue(offB, useA, useB);
         }
 #pragma unroll
         for (int k = 0; k < 8; k++) v += ring[k];
