@@ -84,8 +84,10 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         double ring[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) ring[k] = 0.0;
-        // the offsets of eight pairs (no memory access under a branch)
+        // the offsets of eight pairs (no memory access under a branch);
+        // returns whether any lane has a pair of them within the cutoff
         auto offsets = [&](int j0, unsigned(&off)[8]) {
+            bool hit = false;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
                 const int jb = j0 + 4 * h;
@@ -114,10 +116,12 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
                         const bool in = d2 > 0.0;
                         o = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
                         npairs += in ? 1 : 0;
+                        hit = true;
                     }
                     off[4 * h + k] = o;
                 }
             }
+            return __builtin_amdgcn_ballot_w64(hit) != 0;
         };
         // In pair order, the values of the previous eight pairs into the row
         // sum, and these pairs' loads into the freed slots.  The empty asm
@@ -126,21 +130,33 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         // adds (a load reuses its slot's registers).  `keep`, the registers
         // that the previous loads read their offsets from, stays untouched
         // until here: the compiler makes an instruction that writes a
-        // register which a load in flight reads wait for that load.
-        auto issue = [&](const unsigned(&off)[8], const unsigned(&keep)[8], unsigned(&used)[8]) {
+        // register which a load in flight reads wait for that load.  A group
+        // with no pair within the cutoff in any lane issues no loads (they
+        // would all return +0.0), and its adds are skipped a group later.
+        bool pending = false;  // the ring holds the loads of the previous group
+        auto issue = [&](const unsigned(&off)[8], const unsigned(&keep)[8], unsigned(&used)[8],
+                         bool hit) {
 #pragma unroll
             for (int k = 0; k < 8; k++) asm volatile("" : "+v"(ring[k]) : "v"(off[7]));
+            if (pending) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) v += ring[k];
+                for (int k = 0; k < 8; k++) v += ring[k];
+            }
             asm volatile("" ::"v"(keep[0]), "v"(keep[1]), "v"(keep[2]), "v"(keep[3]),
                          "v"(keep[4]), "v"(keep[5]), "v"(keep[6]), "v"(keep[7]));
 #pragma unroll
-            for (int k = 0; k < 8; k++) {
-                unsigned o = off[k];
-                asm volatile("" : "+v"(o) : "v"(v));
-                ring[k] = __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(tbl, o, 0, 0));
-                used[k] = o;  // the register the load reads
+            for (int k = 0; k < 8; k++) used[k] = off[k];
+            if (hit) {
+#pragma unroll
+                for (int k = 0; k < 8; k++) {
+                    unsigned o = off[k];
+                    asm volatile("" : "+v"(o) : "v"(v));
+                    ring[k] = __builtin_bit_cast(double,
+                                                 __builtin_amdgcn_raw_buffer_load_b64(tbl, o, 0, 0));
+                    used[k] = o;  // the register the load reads
+                }
             }
+            pending = hit;
         };
         unsigned offA[8], offB[8], useA[8], useB[8];
 #pragma unroll
@@ -149,13 +165,15 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         // the loop only the ring's loads are waited for
         __builtin_amdgcn_s_waitcnt(0);
         for (int j0 = 0; j0 < nlp; j0 += 16) {
-            offsets(j0, offA);
-            issue(offA, useB, useA);
-            offsets(j0 + 8, offB);
-            issue(offB, useA, useB);
+            const bool hitA = offsets(j0, offA);
+            issue(offA, useB, useA, hitA);
+            const bool hitB = offsets(j0 + 8, offB);
+            issue(offB, useA, useB, hitB);
         }
+        if (pending) {
 #pragma unroll
-        for (int k = 0; k < 8; k++) v += ring[k];
+            for (int k = 0; k < 8; k++) v += ring[k];
+        }
         double *vb = vbuf + (ch & 1) * kScore64Threads;
         vb[tid] = v;
         __syncthreads();
