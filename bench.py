@@ -280,6 +280,9 @@ def main():
                          "of the tuner's choice")
     ap.add_argument("--ligand-gemm", type=int, choices=(0, 1), default=None,
                     help="ligand displacement GEMM off/on instead of the tuner's choice")
+    ap.add_argument("--profile-fp64-step", action="store_true",
+                    help="only build the workload and run one plain FP64 step of each complex "
+                         "(for rocprofv3 --pmc runs of the FP64 kernel)")
     ap.add_argument("--profile-step", action="store_true",
                     help="only build the workload and run one FP32 step (for rocprofv3 --pmc runs), "
                          "after a 1 GiB device copy used to calibrate FETCH_SIZE/WRITE_SIZE, "
@@ -307,6 +310,20 @@ def main():
             args.variant = names.index(args.variant)
         else:
             sys.exit("unknown variant %r; compiled variants: %s" % (args.variant, ", ".join(names)))
+
+    if args.profile_fp64_step:
+        wl = workloads.build(args.workload, swarm_scale=args.swarm_scale, steps=1)
+        for cx, S, k in zip(wl["complexes"], wl["n_swarms"], range(len(wl["complexes"]))):
+            off = sum(wl["n_swarms"][:k]) * wl["G"] * cx.P
+            r64 = sdgso.Run(cx, S, wl["G"], wl["pos"][off:off + S * wl["G"] * cx.P],
+                            sdgso.stream(0, S, 1, wl["G"]), 1, form=sdgso.FP64)
+            r64.steps(1)
+            r64.sync()
+            r64.close()
+        log("profile FP64 step done: %d complexes, %d all-pairs evaluations"
+            % (len(wl["complexes"]),
+               sum(S * wl["G"] * cx.nr * cx.nl for cx, S in zip(wl["complexes"], wl["n_swarms"]))))
+        return
 
     if args.profile_step:
         x = torch.ones(256 * 1024 * 1024, dtype=torch.float32, device="cuda")
