@@ -3,12 +3,13 @@
 // is the calculate_dfire kernel of source/dock/src/cuda/gpu_baseline:492).
 //
 // Layout per workgroup:
-//   LDS  ligand pose in FP64 SoA (x[nlp], y[nlp], z[nlp]) + type*20 [nlp],
-//        nl padded to nlp = a multiple of 16 with far-away atoms, + two
-//        256-entry row-sum buffers
+//   LDS  ligand pose in FP64 in groups of four atoms (x[4], y[4], z[4],
+//        type*20[4]: 112 bytes), nl padded to nlp = a multiple of 16 with
+//        far-away atoms, + two 256-entry row-sum buffers
 //   lane one receptor row i at a time (i = chunk*256 + tid), its FP64
 //        coordinates displaced in registers; ligand atoms are LDS broadcasts,
-//        two atoms' coordinates per 16-byte read
+//        two atoms' coordinates per 16-byte read, a group's seven reads from
+//        one address
 //
 // A pair's table value is a buffer load whose offset lies past the table
 // (the load returns +0.0) unless the pair is inside (0, 225]; the loads go
@@ -34,11 +35,10 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     const DevComplex &c = a.cx[a.swarm_cx[swarm]];
     const int nl = c.nl, nr = c.nr;
     const int nlp = score64_padded_nl(nl);
-    double *lx = lds;
-    double *ly = lx + nlp;
-    double *lz = ly + nlp;
-    double *vbuf = lz + nlp;                        // [2][256]
-    int *lt = (int *)(vbuf + 2 * kScore64Threads);  // [nlp]
+    // ligand atoms in groups of four, 112 bytes each: x[4], y[4], z[4] in FP64
+    // and the four types, so one address and immediate offsets read a group
+    double *la = lds;                               // [nlp / 4][14]
+    double *vbuf = la + 14 * (nlp / 4);             // [2][256]
 
     const double *pose = a.pos + (size_t)gid * a.P;
     double inv[4];
@@ -50,10 +50,11 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
             ligand_atom64(c, pose, inv, j, x, y, z);
             t = gld(c.ltype, j);
         }
-        lx[j] = x;
-        ly[j] = y;
-        lz[j] = z;
-        lt[j] = t;
+        double *q = la + 14 * (j >> 2) + (j & 3);
+        q[0] = x;
+        q[4] = y;
+        q[8] = z;
+        ((int *)(la + 14 * (j >> 2) + 12))[j & 3] = t;
     }
     if (tid == 0) {
         double r[3];
@@ -90,17 +91,15 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
             bool hit = false;
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const int jb = j0 + 4 * h;
-                const double2 x01 = *(const double2 *)(lx + jb), x23 = *(const double2 *)(lx + jb + 2);
-                const double2 y01 = *(const double2 *)(ly + jb), y23 = *(const double2 *)(ly + jb + 2);
-                const double2 z01 = *(const double2 *)(lz + jb), z23 = *(const double2 *)(lz + jb + 2);
-                const int4 t4 = *(const int4 *)(lt + jb);
+                const double *q = la + 14 * ((j0 >> 2) + h);
+                const double2 x01 = *(const double2 *)(q + 0), x23 = *(const double2 *)(q + 2);
+                const double2 y01 = *(const double2 *)(q + 4), y23 = *(const double2 *)(q + 6);
+                const double2 z01 = *(const double2 *)(q + 8), z23 = *(const double2 *)(q + 10);
+                const int4 t4 = *(const int4 *)(q + 12);
                 const double xs[4] = {x01.x, x01.y, x23.x, x23.y};
                 const double ys[4] = {y01.x, y01.y, y23.x, y23.y};
                 const double zs[4] = {z01.x, z01.y, z23.x, z23.y};
-                int ts[4] = {t4.x, t4.y, t4.z, t4.w};
-                // the types are read here, not under the branches below
-                asm volatile("" : "+v"(ts[0]), "+v"(ts[1]), "+v"(ts[2]), "+v"(ts[3]));
+                const int ts[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
                 for (int k = 0; k < 4; k++) {
                     // pair_bin64 with the test d2 > 0 (coincident atoms, rare)
