@@ -70,7 +70,8 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         (void *)c.table64, (short)0, (SDGSO_TABLE_LEN + 20) * 8, 0x00020000);
     constexpr unsigned kNoLoad64 = 0x80000000u;  // past the table: the load returns +0.0
     double energy = 0.0;  // thread 0 only
-    long long npairs = 0;
+    // a lane's pairs: at most ceil(nr / 256) * nlp, far below 2^32
+    unsigned lane_pairs = 0;
     const int nchunks = (nr + kScore64Threads - 1) / kScore64Threads;
     for (int ch = 0; ch < nchunks; ch++) {
         const int i = ch * kScore64Threads + tid;
@@ -85,6 +86,13 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         double ring[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) ring[k] = 0.0;
+        // The table offsets of two groups of eight pairs, the group in
+        // flight and the next one.  A register holds kNoLoad64 until a pair
+        // within the cutoff writes it and goes back to kNoLoad64 once its
+        // load has been added, so a pair outside the cutoff costs nothing.
+        unsigned offA[8], offB[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) offA[k] = offB[k] = kNoLoad64;
         // the offsets of eight pairs (no memory access under a branch);
         // returns whether any lane has a pair of them within the cutoff
         auto offsets = [&](int j0, unsigned(&off)[8]) {
@@ -109,65 +117,53 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
                     const double dy = ry - ys[k];
                     const double dz = rz - zs[k];
                     const double d2 = dx * dx + dy * dy + dz * dz;
-                    unsigned o = kNoLoad64;
                     if (d2 <= kCutoff2) {
                         const int d = (int)(sqrt(d2) * 2.0 - 1.0);
                         const bool in = d2 > 0.0;
-                        o = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
-                        npairs += in ? 1 : 0;
+                        off[4 * h + k] = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
+                        lane_pairs += in ? 1u : 0u;
                         hit = true;
                     }
-                    off[4 * h + k] = o;
                 }
             }
             return __builtin_amdgcn_ballot_w64(hit) != 0;
         };
         // In pair order, the values of the previous eight pairs into the row
-        // sum, and these pairs' loads into the freed slots.  The empty asm
-        // statements order them: the adds after this group's arithmetic (a
-        // slot's load then has a whole group behind it), the loads after the
-        // adds (a load reuses its slot's registers).  `keep`, the registers
-        // that the previous loads read their offsets from, stays untouched
-        // until here: the compiler makes an instruction that writes a
-        // register which a load in flight reads wait for that load.  A group
-        // with no pair within the cutoff in any lane issues no loads (they
-        // would all return +0.0), and its adds are skipped a group later.
+        // sum (their offsets back to kNoLoad64), and these pairs' loads into
+        // the freed slots.  The empty asm statements order them: the adds
+        // after this group's arithmetic (a slot's load then has a whole group
+        // behind it), the loads after the adds (a load reuses its slot's
+        // registers).  A group with no pair within the cutoff in any lane
+        // issues no loads (they would all return +0.0), and its adds are
+        // skipped a group later.
         bool pending = false;  // the ring holds the loads of the previous group
-        auto issue = [&](const unsigned(&off)[8], const unsigned(&keep)[8], unsigned(&used)[8],
-                         bool hit) {
+        auto issue = [&](unsigned(&off)[8], unsigned(&prev)[8], bool hit) {
 #pragma unroll
             for (int k = 0; k < 8; k++) asm volatile("" : "+v"(ring[k]) : "v"(off[7]));
             if (pending) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) v += ring[k];
-            }
-            asm volatile("" ::"v"(keep[0]), "v"(keep[1]), "v"(keep[2]), "v"(keep[3]),
-                         "v"(keep[4]), "v"(keep[5]), "v"(keep[6]), "v"(keep[7]));
 #pragma unroll
-            for (int k = 0; k < 8; k++) used[k] = off[k];
+                for (int k = 0; k < 8; k++) prev[k] = kNoLoad64;
+            }
             if (hit) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    unsigned o = off[k];
-                    asm volatile("" : "+v"(o) : "v"(v));
-                    ring[k] = __builtin_bit_cast(double,
-                                                 __builtin_amdgcn_raw_buffer_load_b64(tbl, o, 0, 0));
-                    used[k] = o;  // the register the load reads
+                    asm volatile("" : "+v"(off[k]) : "v"(v));
+                    ring[k] = __builtin_bit_cast(
+                        double, __builtin_amdgcn_raw_buffer_load_b64(tbl, off[k], 0, 0));
                 }
             }
             pending = hit;
         };
-        unsigned offA[8], offB[8], useA[8], useB[8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) useB[k] = kNoLoad64;
         // nothing in flight on entry (the row's loads done), so that inside
         // the loop only the ring's loads are waited for
         __builtin_amdgcn_s_waitcnt(0);
         for (int j0 = 0; j0 < nlp; j0 += 16) {
             const bool hitA = offsets(j0, offA);
-            issue(offA, useB, useA, hitA);
+            issue(offA, offB, hitA);
             const bool hitB = offsets(j0 + 8, offB);
-            issue(offB, useA, useB, hitB);
+            issue(offB, offA, hitB);
         }
         if (pending) {
 #pragma unroll
@@ -183,6 +179,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     }
 
     // pair count: wave reduction, then across the 4 waves through LDS
+    long long npairs = lane_pairs;
     for (int off = 32; off > 0; off >>= 1) npairs += __shfl_down(npairs, off, 64);
     __syncthreads();
     long long *wsum = (long long *)vbuf;
