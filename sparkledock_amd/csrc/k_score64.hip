@@ -26,6 +26,26 @@
 
 namespace sd {
 
+// sqrt(x) for 0 < x <= 225, equal to the correctly rounded sqrt for
+// x >= 2^-100: v_rsq_f64, then the Goldschmidt iteration for the root and
+// half its reciprocal with two Newton corrections of the root (P. Markstein,
+// "IA-64 and Elementary Functions", 2000), the steps the AMDGPU backend takes
+// for an FP64 sqrt less its range scaling (for x < 2^-767) and its zero and
+// infinity cases.  Smaller x are raised to 2^-100: every x < 4 has bin 0.
+__device__ __forceinline__ double sqrt_in_cutoff(double x) {
+    // one v_max_f64 (x is a sum of products, never a signalling NaN)
+    asm("v_max_f64 %0, %1, %2" : "=v"(x) : "v"(x), "s"(0x1p-100));
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = y * 0.5;
+    const double r = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r, g);
+    h = __builtin_fma(h, r, h);
+    double e = __builtin_fma(-g, g, x);
+    g = __builtin_fma(e, h, g);
+    e = __builtin_fma(-g, g, x);
+    return __builtin_fma(e, h, g);
+}
+
 __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     extern __shared__ __align__(16) double lds[];
     const int gid = blockIdx.x;
@@ -118,7 +138,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
                     const double dz = rz - zs[k];
                     const double d2 = dx * dx + dy * dy + dz * dz;
                     if (d2 <= kCutoff2) {
-                        const int d = (int)(sqrt(d2) * 2.0 - 1.0);
+                        const int d = (int)(sqrt_in_cutoff(d2) * 2.0 - 1.0);
                         const bool in = d2 > 0.0;
                         off[4 * h + k] = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
                         lane_pairs += in ? 1u : 0u;
