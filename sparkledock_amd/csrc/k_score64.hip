@@ -115,11 +115,11 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         for (int k = 0; k < 8; k++) offA[k] = offB[k] = kNoLoad64;
         // the offsets of eight pairs (no memory access under a branch);
         // returns whether any lane has a pair of them within the cutoff
-        auto offsets = [&](int j0, unsigned(&off)[8]) {
-            bool hit = false;
+        auto offsets = [&](const double *g8, unsigned(&off)[8]) {
+            unsigned long long near_lanes = 0;  // lanes with a pair within the cutoff
 #pragma unroll
             for (int h = 0; h < 2; h++) {
-                const double *q = la + 14 * ((j0 >> 2) + h);
+                const double *q = g8 + 14 * h;
                 const double2 x01 = *(const double2 *)(q + 0), x23 = *(const double2 *)(q + 2);
                 const double2 y01 = *(const double2 *)(q + 4), y23 = *(const double2 *)(q + 6);
                 const double2 z01 = *(const double2 *)(q + 8), z23 = *(const double2 *)(q + 10);
@@ -137,16 +137,17 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
                     const double dy = ry - ys[k];
                     const double dz = rz - zs[k];
                     const double d2 = dx * dx + dy * dy + dz * dz;
-                    if (d2 <= kCutoff2) {
+                    const bool near = d2 <= kCutoff2;
+                    near_lanes |= __builtin_amdgcn_ballot_w64(near);
+                    if (near) {
                         const int d = (int)(sqrt_in_cutoff(d2) * 2.0 - 1.0);
                         const bool in = d2 > 0.0;
                         off[4 * h + k] = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
                         lane_pairs += in ? 1u : 0u;
-                        hit = true;
                     }
                 }
             }
-            return __builtin_amdgcn_ballot_w64(hit) != 0;
+            return near_lanes != 0;
         };
         // In pair order, the values of the previous eight pairs into the row
         // sum (their offsets back to kNoLoad64), and these pairs' loads into
@@ -179,10 +180,15 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         // nothing in flight on entry (the row's loads done), so that inside
         // the loop only the ring's loads are waited for
         __builtin_amdgcn_s_waitcnt(0);
-        for (int j0 = 0; j0 < nlp; j0 += 16) {
-            const bool hitA = offsets(j0, offA);
+        // the byte offset of the sixteen atoms in a VGPR: their 28 reads
+        // take it with immediate offsets, one add per sixteen atoms
+        unsigned gb = 0;
+        for (int j0 = 0; j0 < nlp; j0 += 16, gb += 16 * 28) {
+            asm("" : "+v"(gb));
+            const double *g16 = (const double *)((const char *)la + gb);
+            const bool hitA = offsets(g16, offA);
             issue(offA, offB, hitA);
-            const bool hitB = offsets(j0 + 8, offB);
+            const bool hitB = offsets(g16 + 28, offB);
             issue(offB, offA, hitB);
         }
         if (pending) {
