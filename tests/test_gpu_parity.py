@@ -272,6 +272,44 @@ def test_fp32_guard_sweep_across_every_boundary():
         assert np.array_equal(per_bin_counts(pr, poses, None, v)[:, 0], np.array(want)), v
 
 
+def test_tiny_distances_take_bin_zero():
+    """Ligand atoms from 0.3 A down to 1e-160 A (d2 subnormal) and at 0 from a
+    receptor atom: every 0 < d2 < 4 has bin 0 (sqrt(d2)*2 - 1 rounds to -1
+    below about 3e-33, which the oracle maps to bin 0), d2 = 0 is no pair.
+    The FP64 kernel's root skips the range scaling below 2^-767 and raises d2
+    below 2^-100 to 2^-100; both forms must still match the oracle."""
+    rng = np.random.default_rng(23)
+    dist = np.array([0.3, 1e-1, 1e-5, 1e-10, 1e-16, 1e-17, 1e-20, 1e-30, 1e-54, 1e-60, 1e-100,
+                     1e-154, 1e-155, 1e-160, 0.0, 1.9, 2.1, 7.49, 7.51])
+    n = len(dist)
+    dirs = rng.normal(size=(n, 3))
+    dirs /= np.linalg.norm(dirs, axis=1, keepdims=True)
+    c = dict(r_xyz=np.array([[0.0, 0.0, 0.0], [400.0, 0.0, 0.0]]),
+             l_xyz=dirs * dist[:, None],
+             r_type=np.array([5, 9], np.int32), l_type=rng.integers(0, 167, n).astype(np.int32),
+             table=(rng.normal(size=168 * 168 * 20) * 0.5).astype(np.float32),
+             r_modes=np.zeros((0, 2, 3)), l_modes=np.zeros((0, n, 3)),
+             r_mask=np.ones(2, np.int32), l_mask=np.ones(n, np.int32),
+             lig_ref=np.zeros(3))
+    c["pos"] = np.array([[[0.0, 0.0, 0.0, 1.0, 0.0, 0.0, 0.0]]])
+    pr = GC.Pair(c)
+    poses = poses_of(c)
+    s0, _, p0 = pr.oracle_scores(poses)
+    assert p0[0] == n - 1  # every atom but the one at the origin
+    s, _, p = pr.gpu.score(poses, sdgso.FP64)
+    assert np.array_equal(p, p0) and np.array_equal(s, s0)
+    s, _, p = pr.gpu.score(poses, sdgso.FP32_GUARD)
+    assert np.array_equal(p, p0) and abs(s[0] - s0[0]) < 1e-5
+    want = []
+    for b in range(20):
+        t = np.zeros((168, 168, 20), np.float32)
+        t[:, :, b] = 1.0
+        want.append(round((4.7 - pr.with_table(t.reshape(-1)).score_pose(poses[0])[0]) / 0.0157))
+    assert want[0] == 15  # the fifteen atoms with 0 < d < 2 A
+    for form in (sdgso.FP64, sdgso.FP32_GUARD):
+        assert np.array_equal(per_bin_counts(pr, poses, form)[:, 0], np.array(want)), form
+
+
 def large_displacement_case(seed=19, nm=10, ext=25.0):
     """Receptor atom 0 displaced by ten modes with extents of tens of A, so
     that its FP32 displacement is off by a few 1e-6 A and d2 by more than the
