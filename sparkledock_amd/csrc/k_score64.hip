@@ -5,16 +5,17 @@
 // Layout per workgroup:
 //   LDS  ligand pose in FP64 in groups of four atoms (x[4], y[4], z[4],
 //        type*20[4]: 112 bytes), nl padded to nlp = a multiple of 16 with
-//        far-away atoms, + two 256-entry row-sum buffers
-//   lane one receptor row i at a time (i = chunk*256 + tid), its FP64
+//        far-away atoms, + two 512-entry row-sum buffers
+//   lane two receptor rows at a time (base + tid and base + 256 + tid) while
+//        512-row chunks are full, then one (base + tid), their FP64
 //        coordinates displaced in registers; ligand atoms are LDS broadcasts,
 //        two atoms' coordinates per 16-byte read, a group's seven reads from
-//        one address
+//        one address, each atom read once for the lane's rows
 //
 // A pair's table value is a buffer load whose offset lies past the table
 // (the load returns +0.0) unless the pair is inside (0, 225]; the loads go
-// into a ring of eight and each is added to the row sum eight pairs later, so
-// a gather's latency is not waited for at its pair.  The additions stay in
+// into a ring of eight and each is added to its row sum a group of eight
+// pairs later, so a gather's latency is not waited for at its pair.  The additions stay in
 // pair order, and adding +0.0 leaves an FP64 sum unchanged (a row sum that
 // starts at +0.0 is never -0.0).
 // The energy is summed exactly as prepare_gso_tasks.cc:1539-1570 does: each
@@ -46,7 +47,8 @@ __device__ __forceinline__ double sqrt_in_cutoff(double x) {
     return __builtin_fma(e, h, g);
 }
 
-__global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
+// five waves per SIMD: 96 VGPRs (98 unbounded; two spilled outside the pair loop)
+__global__ __launch_bounds__(kScore64Threads) __attribute__((amdgpu_waves_per_eu(5, 8))) void score64_kernel(ScoreArgs a) {
     extern __shared__ __align__(16) double lds[];
     const int gid = blockIdx.x;
     if (gid >= a.n_glow) return;
@@ -58,7 +60,7 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     // ligand atoms in groups of four, 112 bytes each: x[4], y[4], z[4] in FP64
     // and the four types, so one address and immediate offsets read a group
     double *la = lds;                               // [nlp / 4][14]
-    double *vbuf = la + 14 * (nlp / 4);             // [2][256]
+    double *vbuf = la + 14 * (nlp / 4);             // [2][512]
 
     const double *pose = a.pos + (size_t)gid * a.P;
     double inv[4];
@@ -92,17 +94,28 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
     double energy = 0.0;  // thread 0 only
     // a lane's pairs: at most ceil(nr / 256) * nlp, far below 2^32
     unsigned lane_pairs = 0;
-    const int nchunks = (nr + kScore64Threads - 1) / kScore64Threads;
-    for (int ch = 0; ch < nchunks; ch++) {
-        const int i = ch * kScore64Threads + tid;
+    int nchunk = 0;  // chunks so far: the row-sum buffer alternates
+    // One chunk of R * 256 receptor rows from `base`: lane tid takes rows
+    // base + r * 256 + tid (r < R), so that one broadcast read of a ligand
+    // atom serves R rows.  A group is eight pairs: 8 / R atoms x R rows.
+    auto chunk = [&](auto rows_per_lane, int base) {
+        constexpr int R = decltype(rows_per_lane)::value;
         // rows past nr sit far away and find no pair, so every lane runs the loop
-        double rx = -1e30, ry = -1e30, rz = -1e30;
-        int ra = 0;
-        if (i < nr) {
-            receptor_atom64(c, ext, i, rx, ry, rz);
-            ra = gld(c.rtype, i);
+        double rx[R], ry[R], rz[R], v[R];
+        int ra[R];
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+            const int i = base + r * kScore64Threads + tid;
+            rx[r] = -1e30;
+            ry[r] = -1e30;
+            rz[r] = -1e30;
+            ra[r] = 0;
+            v[r] = 0.0;
+            if (i < nr) {
+                receptor_atom64(c, ext, i, rx[r], ry[r], rz[r]);
+                ra[r] = gld(c.rtype, i);
+            }
         }
-        double v = 0.0;
         double ring[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) ring[k] = 0.0;
@@ -113,13 +126,14 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         unsigned offA[8], offB[8];
 #pragma unroll
         for (int k = 0; k < 8; k++) offA[k] = offB[k] = kNoLoad64;
-        // the offsets of eight pairs (no memory access under a branch);
-        // returns whether any lane has a pair of them within the cutoff
-        auto offsets = [&](const double *g8, unsigned(&off)[8]) {
+        // the offsets of a group, slot 4 * (h + r) + k for atom 4 h + k of
+        // the group and row r (no memory access under a branch); returns
+        // whether any lane has a pair of them within the cutoff
+        auto offsets = [&](const double *g, unsigned(&off)[8]) {
             unsigned long long near_lanes = 0;  // lanes with a pair within the cutoff
 #pragma unroll
-            for (int h = 0; h < 2; h++) {
-                const double *q = g8 + 14 * h;
+            for (int h = 0; h < 2 / R; h++) {
+                const double *q = g + 14 * h;
                 const double2 x01 = *(const double2 *)(q + 0), x23 = *(const double2 *)(q + 2);
                 const double2 y01 = *(const double2 *)(q + 4), y23 = *(const double2 *)(q + 6);
                 const double2 z01 = *(const double2 *)(q + 8), z23 = *(const double2 *)(q + 10);
@@ -129,55 +143,58 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
                 const double zs[4] = {z01.x, z01.y, z23.x, z23.y};
                 const int ts[4] = {t4.x, t4.y, t4.z, t4.w};
 #pragma unroll
-                for (int k = 0; k < 4; k++) {
-                    // pair_bin64 with the test d2 > 0 (coincident atoms, rare)
-                    // under the cutoff branch: one compare less per pair
-                    const int row = ra + ts[k];
-                    const double dx = rx - xs[k];
-                    const double dy = ry - ys[k];
-                    const double dz = rz - zs[k];
-                    const double d2 = dx * dx + dy * dy + dz * dz;
-                    const bool near = d2 <= kCutoff2;
-                    near_lanes |= __builtin_amdgcn_ballot_w64(near);
-                    if (near) {
-                        const int d = (int)(sqrt_in_cutoff(d2) * 2.0 - 1.0);
-                        const bool in = d2 > 0.0;
-                        off[4 * h + k] = in ? 8u * (unsigned)(row + bin_of_d(d)) : kNoLoad64;
-                        lane_pairs += in ? 1u : 0u;
+                for (int r = 0; r < R; r++) {
+#pragma unroll
+                    for (int k = 0; k < 4; k++) {
+                        // pair_bin64 with the test d2 > 0 (coincident atoms,
+                        // rare) under the cutoff branch: one compare less per pair
+                        const double dx = rx[r] - xs[k];
+                        const double dy = ry[r] - ys[k];
+                        const double dz = rz[r] - zs[k];
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        const bool near = d2 <= kCutoff2;
+                        near_lanes |= __builtin_amdgcn_ballot_w64(near);
+                        if (near) {
+                            const int d = (int)(sqrt_in_cutoff(d2) * 2.0 - 1.0);
+                            const bool in = d2 > 0.0;
+                            off[4 * (h + r) + k] =
+                                in ? 8u * (unsigned)(ra[r] + ts[k] + bin_of_d(d)) : kNoLoad64;
+                            lane_pairs += in ? 1u : 0u;
+                        }
                     }
                 }
             }
             return near_lanes != 0;
         };
-        // In pair order, the values of the previous eight pairs into the row
-        // sum (their offsets back to kNoLoad64), and these pairs' loads into
-        // the freed slots.  The empty asm statements order them: the adds
-        // after this group's arithmetic (a slot's load then has a whole group
-        // behind it), the loads after the adds (a load reuses its slot's
-        // registers).  A group with no pair within the cutoff in any lane
-        // issues no loads (they would all return +0.0), and its adds are
-        // skipped a group later.
+        // In pair order within each row, the values of the previous group
+        // into the row sums (their offsets back to kNoLoad64), and this
+        // group's loads into the freed slots.  The empty asm statements order
+        // them: the adds after this group's arithmetic (a slot's load then
+        // has a whole group behind it), the loads after the adds (a load
+        // reuses its slot's registers).  A group with no pair within the
+        // cutoff in any lane issues no loads (they would all return +0.0),
+        // and its adds are skipped a group later.
         bool pending = false;  // the ring holds the loads of the previous group
         auto issue = [&](unsigned(&off)[8], unsigned(&prev)[8], bool hit) {
 #pragma unroll
             for (int k = 0; k < 8; k++) asm volatile("" : "+v"(ring[k]) : "v"(off[7]));
             if (pending) {
 #pragma unroll
-                for (int k = 0; k < 8; k++) v += ring[k];
+                for (int k = 0; k < 8; k++) v[k / (8 / R)] += ring[k];
 #pragma unroll
                 for (int k = 0; k < 8; k++) prev[k] = kNoLoad64;
             }
             if (hit) {
 #pragma unroll
                 for (int k = 0; k < 8; k++) {
-                    asm volatile("" : "+v"(off[k]) : "v"(v));
+                    asm volatile("" : "+v"(off[k]) : "v"(v[R - 1]));
                     ring[k] = __builtin_bit_cast(
                         double, __builtin_amdgcn_raw_buffer_load_b64(tbl, off[k], 0, 0));
                 }
             }
             pending = hit;
         };
-        // nothing in flight on entry (the row's loads done), so that inside
+        // nothing in flight on entry (the rows' loads done), so that inside
         // the loop only the ring's loads are waited for
         __builtin_amdgcn_s_waitcnt(0);
         // the byte offset of the sixteen atoms in a VGPR: their 28 reads
@@ -186,23 +203,33 @@ __global__ __launch_bounds__(kScore64Threads) void score64_kernel(ScoreArgs a) {
         for (int j0 = 0; j0 < nlp; j0 += 16, gb += 16 * 28) {
             asm("" : "+v"(gb));
             const double *g16 = (const double *)((const char *)la + gb);
-            const bool hitA = offsets(g16, offA);
-            issue(offA, offB, hitA);
-            const bool hitB = offsets(g16 + 28, offB);
-            issue(offB, offA, hitB);
+#pragma unroll
+            for (int gi = 0; gi < 2 * R; gi += 2) {
+                const bool hitA = offsets(g16 + 14 * (2 / R) * gi, offA);
+                issue(offA, offB, hitA);
+                const bool hitB = offsets(g16 + 14 * (2 / R) * (gi + 1), offB);
+                issue(offB, offA, hitB);
+            }
         }
         if (pending) {
 #pragma unroll
-            for (int k = 0; k < 8; k++) v += ring[k];
+            for (int k = 0; k < 8; k++) v[k / (8 / R)] += ring[k];
         }
-        double *vb = vbuf + (ch & 1) * kScore64Threads;
-        vb[tid] = v;
+        double *vb = vbuf + (nchunk & 1) * (2 * kScore64Threads);
+#pragma unroll
+        for (int r = 0; r < R; r++) vb[r * kScore64Threads + tid] = v[r];
         __syncthreads();
         if (tid == 0) {
-            const int rows = min(kScore64Threads, nr - ch * kScore64Threads);
+            const int rows = min(R * kScore64Threads, nr - base);
             for (int t = 0; t < rows; t++) energy += vb[t];
         }
-    }
+        nchunk++;
+    };
+    // chunks of 512 rows while they are full, then of 256
+    int base = 0;
+    for (; nr - base >= 2 * kScore64Threads; base += 2 * kScore64Threads)
+        chunk(std::integral_constant<int, 2>{}, base);
+    for (; base < nr; base += kScore64Threads) chunk(std::integral_constant<int, 1>{}, base);
 
     // pair count: wave reduction, then across the 4 waves through LDS
     long long npairs = lane_pairs;
