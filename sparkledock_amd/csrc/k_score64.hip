@@ -5,7 +5,7 @@
 // Layout per workgroup:
 //   LDS  ligand pose in FP64 in groups of four atoms (x[4], y[4], z[4],
 //        type*20[4]: 112 bytes), nl padded to nlp = a multiple of 16 with
-//        far-away atoms, + two 512-entry row-sum buffers
+//        far-away atoms, + two 256-entry row-sum buffers
 //   lane two receptor rows at a time (base + tid and base + 256 + tid) while
 //        512-row chunks are full, then one (base + tid), their FP64
 //        coordinates displaced in registers; ligand atoms are LDS broadcasts,
@@ -60,7 +60,7 @@ __global__ __launch_bounds__(kScore64Threads) __attribute__((amdgpu_waves_per_eu
     // ligand atoms in groups of four, 112 bytes each: x[4], y[4], z[4] in FP64
     // and the four types, so one address and immediate offsets read a group
     double *la = lds;                               // [nlp / 4][14]
-    double *vbuf = la + 14 * (nlp / 4);             // [2][512]
+    double *vbuf = la + 14 * (nlp / 4);             // [2][256]
 
     const double *pose = a.pos + (size_t)gid * a.P;
     double inv[4];
@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kScore64Threads) __attribute__((amdgpu_waves_per_eu
     double energy = 0.0;  // thread 0 only
     // a lane's pairs: at most ceil(nr / 256) * nlp, far below 2^32
     unsigned lane_pairs = 0;
-    int nchunk = 0;  // chunks so far: the row-sum buffer alternates
+    int nchunk = 0;  // 256-row halves so far: the row-sum buffer alternates
     // One chunk of R * 256 receptor rows from `base`: lane tid takes rows
     // base + r * 256 + tid (r < R), so that one broadcast read of a ligand
     // atom serves R rows.  A group is eight pairs: 8 / R atoms x R rows.
@@ -215,15 +215,20 @@ __global__ __launch_bounds__(kScore64Threads) __attribute__((amdgpu_waves_per_eu
 #pragma unroll
             for (int k = 0; k < 8; k++) v[k / (8 / R)] += ring[k];
         }
-        double *vb = vbuf + (nchunk & 1) * (2 * kScore64Threads);
+        // each 256-row half through the double buffer in turn: a half is
+        // written only after the barrier that follows thread 0's sum of the
+        // half before the previous one
 #pragma unroll
-        for (int r = 0; r < R; r++) vb[r * kScore64Threads + tid] = v[r];
-        __syncthreads();
-        if (tid == 0) {
-            const int rows = min(R * kScore64Threads, nr - base);
-            for (int t = 0; t < rows; t++) energy += vb[t];
+        for (int r = 0; r < R; r++) {
+            double *vb = vbuf + (nchunk & 1) * kScore64Threads;
+            vb[tid] = v[r];
+            __syncthreads();
+            if (tid == 0) {
+                const int rows = min(kScore64Threads, nr - base - r * kScore64Threads);
+                for (int t = 0; t < rows; t++) energy += vb[t];
+            }
+            nchunk++;
         }
-        nchunk++;
     };
     // chunks of 512 rows while they are full, then of 256
     int base = 0;

@@ -63,11 +63,11 @@ constexpr int kStatSlots = 256;        // slots per filter-statistics counter
 constexpr int kScore64Threads = 256;   // FP64 form: threads per workgroup (one glowworm)
 
 // FP64 form: ligand atoms padded to a multiple of 16 (far-away atoms), and its
-// LDS: FP64 x, y, z and an int type per padded atom, two 512-row sum buffers.
+// LDS: FP64 x, y, z and an int type per padded atom, two 256-row sum buffers.
 inline __host__ __device__ int score64_padded_nl(int nl) { return (nl + 15) & ~15; }
 inline size_t score64_lds_bytes(int max_nl) {
     const size_t nlp = (size_t)score64_padded_nl(max_nl);
-    return sizeof(double) * (3 * nlp + 4 * kScore64Threads) + sizeof(int) * nlp;
+    return sizeof(double) * (3 * nlp + 2 * kScore64Threads) + sizeof(int) * nlp;
 }
 
 // float4 per receptor slot that hold its FP32 modes (rmodA): 3*nm floats in

@@ -10,7 +10,7 @@ workloads (sparkledock_amd/workloads.py), through the C-ABI.
       four-wave select/move workgroup) from an identical state;
   c4  32 complexes, 300 glowworms, 20+20 modes: a full FP64 step of one
       swarm of the complex with the most pairs (26417/2298 atoms: the FP64
-      kernel's ligand and row sums in 79 KB of LDS, above the 64 KB default, and a
+      kernel's ligand and row sums in 67 KB of LDS, above the 64 KB default, and a
       select/move workgroup of 300 x 47-value poses), and the whole FP32 scoring
       launch with a one-block variant, whose 130 M-wave work list is cut at
       2^26 - 4 waves, checked on the glowworms at the cut, and with the
@@ -257,7 +257,7 @@ def test_c4_largest_complex_fp64_step_g300():
     """One swarm of the c4 complex with the most pairs (26417/2298 atoms,
     20+20 modes, 47-value poses) at G = 300: a whole FP64 step from the same
     state with spread vision ranges.  The FP64 kernel holds 2298 ligand atoms
-    and its row sums in 79 KB of LDS, and select/move moves 300 poses of 47 values; every output
+    and its row sums in 67 KB of LDS, and select/move moves 300 poses of 47 values; every output
     bit-exact but the quaternions (1e-13)."""
     k = 16
     inp, pos = c4_complex(k, 1)
